@@ -1,0 +1,127 @@
+"""ctypes binding of libksh.so (declarations mirror include/ksh.h)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libksh.so")
+
+SUCCESS, NOT_CONVERGED, ASSEMBLY_ERROR, SOLVER_ERROR, NOT_STARTED, OTHER_ERROR = range(6)
+STATUS_NAMES = ["Success", "NotConverged", "AssemblyError", "SolverError", "NotStarted", "OtherError"]
+RES_STATIC, RES_ALM = 0, 1
+FP64, FP32_IR = 0, 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_i64p = C.POINTER(C.c_int64)
+
+
+class Patch(C.Structure):
+    _fields_ = [("p", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("ctrl", _dp)]
+
+
+class Material(C.Structure):
+    _fields_ = [("t", C.c_double), ("E", C.c_double), ("nu", C.c_double), ("rho", C.c_double)]
+
+
+class BC(C.Structure):
+    _fields_ = [("fixed", _ip)]
+
+
+class Loads(C.Structure):
+    _fields_ = [("npoint", C.c_int), ("point_xi", _dp), ("point_F", _dp), ("fsurf", C.c_double * 3)]
+
+
+class AlmOptions(C.Structure):
+    _fields_ = [("method", C.c_int), ("precision", C.c_int), ("nsteps", C.c_int), ("length", C.c_double),
+                ("max_iter", C.c_int), ("tol", C.c_double), ("tol_u", C.c_double), ("tol_f", C.c_double),
+                ("scaling", C.c_double), ("angle_method", C.c_int), ("bifurcation_method", C.c_int),
+                ("relaxation", C.c_double), ("quasi", C.c_int), ("quasi_iterations", C.c_int),
+                ("adaptive", C.c_int), ("adaptive_iterations", C.c_int), ("max_refine", C.c_int),
+                ("refine_tol", C.c_double), ("max_halvings", C.c_int), ("verbose", C.c_int)]
+
+
+class AlmResult(C.Structure):
+    _fields_ = [("steps_done", C.c_int), ("halvings", C.c_int), ("final_status", C.c_int),
+                ("lambda_", _dp), ("unorm", _dp), ("indicator", _dp), ("negatives", _i64p),
+                ("iterations", _ip), ("residue_f", _dp), ("length", _dp), ("U", _dp), ("w", _dp),
+                ("probe_dof", C.c_int), ("wall_seconds", C.c_double), ("phi", C.c_double),
+                ("jacobians", C.c_int64), ("residuals", C.c_int64), ("solves", C.c_int64)]
+
+
+class StaticOptions(C.Structure):
+    _fields_ = [("precision", C.c_int), ("nsteps", C.c_int), ("dlambda", C.c_double), ("max_iter", C.c_int),
+                ("tol", C.c_double), ("max_refine", C.c_int), ("refine_tol", C.c_double), ("verbose", C.c_int)]
+
+
+class StaticResult(C.Structure):
+    _fields_ = [("steps_done", C.c_int), ("final_status", C.c_int), ("lambda_", _dp), ("unorm", _dp),
+                ("iterations", _ip), ("negatives", _i64p), ("residual", _dp), ("U", _dp),
+                ("wall_seconds", C.c_double), ("jacobians", C.c_int64)]
+
+
+# every symbol declared in include/ksh.h: name -> (restype, argtypes)
+_VP = C.c_void_p
+SIGNATURES = {
+    "ksh_last_error": (C.c_char_p, []),
+    "ksh_device_count": (C.c_int, [_ip]),
+    "ksh_problem_create": (C.c_int, [C.POINTER(Patch), C.POINTER(Material), C.POINTER(BC), C.POINTER(Loads),
+                                     C.POINTER(_VP)]),
+    "ksh_problem_destroy": (C.c_int, [_VP]),
+    "ksh_num_free_dofs": (C.c_int, [_VP, _i64p]),
+    "ksh_bandwidth": (C.c_int, [_VP, _i64p]),
+    "ksh_synchronize": (C.c_int, [_VP]),
+    "ksh_assemble_force": (C.c_int, [_VP, _dp]),
+    "ksh_assemble_residual": (C.c_int, [_VP, _dp, C.c_double, C.c_int, _dp]),
+    "ksh_assemble_residual_dev": (C.c_int, [_VP, _VP, C.c_double, C.c_int, _VP]),
+    "ksh_matrix_create": (C.c_int, [_VP, C.c_int, C.POINTER(_VP)]),
+    "ksh_matrix_destroy": (C.c_int, [_VP]),
+    "ksh_matrix_dims": (C.c_int, [_VP, _i64p, _i64p, _ip]),
+    "ksh_assemble_tangent": (C.c_int, [_VP, _dp, _VP]),
+    "ksh_assemble_tangent_dev": (C.c_int, [_VP, _VP, _VP]),
+    "ksh_assemble_mass": (C.c_int, [_VP, _VP]),
+    "ksh_matrix_download": (C.c_int, [_VP, _dp]),
+    "ksh_matrix_upload": (C.c_int, [_VP, _dp]),
+    "ksh_matrix_axpby": (C.c_int, [_VP, C.c_double, C.c_double, _VP]),
+    "ksh_matrix_copy": (C.c_int, [_VP, _VP]),
+    "ksh_matvec": (C.c_int, [_VP, _dp, _dp]),
+    "ksh_matvec_dev": (C.c_int, [_VP, _VP, _VP]),
+    "ksh_factor": (C.c_int, [_VP]),
+    "ksh_solve": (C.c_int, [_VP, _dp, _dp, C.c_int, C.c_double]),
+    "ksh_solve_dev": (C.c_int, [_VP, _VP, _VP, C.c_int, C.c_double]),
+    "ksh_inertia": (C.c_int, [_VP, _i64p, _dp]),
+    "ksh_vector_d": (C.c_int, [_VP, _dp]),
+    "ksh_set_inertia_guard": (C.c_int, [_VP, C.c_double]),
+    "ksh_run_alm": (C.c_int, [_VP, C.POINTER(AlmOptions), C.POINTER(AlmResult)]),
+    "ksh_run_static": (C.c_int, [_VP, C.POINTER(StaticOptions), C.POINTER(StaticResult)]),
+}
+
+_lib = None
+
+
+class KshError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS_NAMES[code] if 0 <= code < 6 else code}: {msg}")
+        self.code = code
+
+
+def load(path: str = LIB_PATH):
+    """Load libksh.so; raises if it is missing (there is no non-GPU execution path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise ImportError(f"libksh.so not built ({path}); run __graft_entry__.build()")
+        L = C.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(code: int):
+    if code != SUCCESS:
+        raise KshError(code, load().ksh_last_error().decode(errors="replace"))
+    return code

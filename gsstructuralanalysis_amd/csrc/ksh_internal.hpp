@@ -1,0 +1,110 @@
+// ksh_internal.hpp -- device-side data layout of one shell problem and one band matrix.
+//
+// HBM layout (one problem):
+//   ctrl0     [ncp][3]            undeformed control net, FP64
+//   free_idx  [ndof]              full dof -> free index or -1
+//   free2full [nfree]             free index -> full dof
+//   tabx/taby [n][nq][3][p+1]     1D basis values / 1st / 2nd parametric derivatives at the
+//                                 Gauss points of each element column / row
+//   wq        [nq*nq]             tensor Gauss weight of a point inside an element (incl. 1/(nx ny))
+//   gpgeo     [nel][nq*nq][16]    undeformed geometry per Gauss point: wdA, SvK matrix C (6),
+//                                 metric A_ab (3), curvature B_ab (3)
+//   u_full    [ndof]              scratch: constructSolution of the current free vector
+//   Ke        [nel][9][16][16]    element tangents in (component pair, A, B) tiles (FP64 or FP32)
+//   Fe        [nel][3*nb]         element internal-force vectors
+//   F         [nfree]             external load (lambda = 1)
+// Band matrix: lower band, column major, ab[J*(bw+1) + (I-J)], FP64 or FP32.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/ksh.h"
+
+namespace ksh {
+
+constexpr int kGeoStride = 16;   // doubles per Gauss point in gpgeo
+constexpr int kTile = 16;        // element tile edge (basis functions per element padded to 16)
+constexpr int kKeTile = 9 * kTile * kTile;
+
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define KSH_HIP(call)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return ::ksh::fail(KSH_OTHER_ERROR, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct Problem {
+    int p = 0, nx = 0, ny = 0, mx = 0, my = 0, nq = 0, nb = 0, ne = 0;
+    int ncp = 0, ndof = 0, nfree = 0, bw = 0, nel = 0, ngp = 0;
+    double t = 0, E = 0, nu = 0, rho = 0;
+    int npoint = 0;
+    double fsurf[3] = {0, 0, 0};
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    double *ctrl0 = nullptr;
+    int *free_idx = nullptr;
+    int *free2full = nullptr;
+    double *tabx = nullptr, *taby = nullptr;
+    double *wq = nullptr;
+    double *gpgeo = nullptr;
+    double *u_full = nullptr;
+    double *u_free = nullptr;      // device staging of host free vectors
+    double *r_free = nullptr;      // device staging of host results
+    void *Ke = nullptr;            // FP64 or FP32 element tiles (sized for FP64)
+    double *Fe = nullptr;
+    double *F = nullptr;
+    double *pl_N = nullptr;        // [npoint][2][p+1] basis values at the load points
+    int *pl_span = nullptr;        // [npoint][2] knot spans
+    double *pl_F = nullptr;        // [npoint][3] load vectors
+};
+
+struct Matrix {
+    Problem *pb = nullptr;
+    int n = 0, bw = 0, prec = KSH_FP64;
+    size_t ld = 0;                 // bw + 1
+    void *ab = nullptr;            // assembled values (FP64 or FP32)
+    void *fac = nullptr;           // L and D after factorization (same precision)
+    double *fac64 = nullptr;       // FP64 refactorization used by the inertia guard
+    double *u_at = nullptr;        // free vector the matrix was assembled at (FP32_IR guard)
+    bool has_u = false;
+    bool factored = false;
+    uint64_t version = 0;          // bumped on every change of ab
+    uint64_t factored_version = ~0ull;
+    int info = 0;                  // 0 or 1-based index of a zero / non-finite pivot
+    double guard = 1e-6;
+    // device workspace for solves and inertia
+    double *work = nullptr;        // 4*n doubles
+    int *flag = nullptr;           // small device int buffer
+    double *red = nullptr;         // reduction scratch
+};
+
+// kernels / launchers (ksh_assembly.hip, ksh_band.hip)
+int launch_setup_geometry(Problem &pb);
+int launch_external_force(Problem &pb);
+int launch_construct_solution(Problem &pb, const double *u_free_dev);
+int launch_residual(Problem &pb, double lambda, int conv, double *R_dev);
+int launch_tangent(Problem &pb, Matrix &K);
+int launch_mass(Problem &pb, Matrix &M);
+
+int band_factor(Matrix &K, void *fac, int prec);
+int band_solve(Matrix &K, const void *fac, int prec, double *x_dev);
+int band_symv(const Matrix &K, const double *x_dev, double *y_dev);
+int band_inertia(Matrix &K, const void *fac, int prec, int64_t *neg, double *dmin, double *dmaxabs,
+                 double *dminabs);
+int band_axpby(Matrix &Y, double alpha, double beta, const Matrix &X);
+int band_convert(const void *src, int src_prec, void *dst, int dst_prec, size_t count, hipStream_t s);
+int vec_axpy(hipStream_t s, int n, double a, const double *x, double *y);
+int vec_sub(hipStream_t s, int n, const double *a, const double *b, double *out);
+int vec_norm2(hipStream_t s, int n, const double *x, double *scratch, double *host_out);
+
+}  // namespace ksh
+
+struct ksh_problem : ksh::Problem {};
+struct ksh_matrix : ksh::Matrix {};

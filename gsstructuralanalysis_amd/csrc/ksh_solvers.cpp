@@ -1,0 +1,163 @@
+// ksh_solvers.cpp -- C-ABI drivers that run the solver classes on the device operator surface.
+//
+// ksh_run_alm    : benchmarks/benchmark_Roof.cpp:320-512 (options, initialize, step loop,
+//                  halve the arc length and restore (Uold, Lold) on NotConverged/AssemblyError)
+// ksh_run_static : examples/example_DisplacementControl.cpp:236-304 (K_L, F, gsStaticNewton with
+//                  the load-controlled residual, gsControlDisplacement steps)
+#include <chrono>
+#include <cstring>
+#include <exception>
+
+#include "gs/gsALMCrisfield.h"
+#include "gs/gsALMRiks.h"
+#include "gs/gsStaticNewton.h"
+#include "gs/ksh_ops.h"
+#include "ksh_internal.hpp"
+
+using namespace gismo;
+
+namespace {
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+extern "C" int ksh_run_alm(ksh_problem *pb, const ksh_alm_options *o, ksh_alm_result *res)
+{
+    if (!pb || !o || !res) return ksh::fail(KSH_OTHER_ERROR, "null argument");
+    try {
+        gsSparseSolver<real_t>::settings().max_refine = o->max_refine;
+        gsSparseSolver<real_t>::settings().refine_tol = o->refine_tol;
+        gsShellOps ops(pb, o->precision);
+        const index_t n = ops.numDofs();
+        gsVector<real_t> Force = ops.force();
+        auto Jacobian = ops.jacobian();
+        auto ALResidual = ops.alResidual();
+
+        std::unique_ptr<gsALMBase<real_t>> alm;
+        if (o->method == 0) alm.reset(new gsALMCrisfield<real_t>(Jacobian, ALResidual, Force));
+        else if (o->method == 1) alm.reset(new gsALMRiks<real_t>(Jacobian, ALResidual, Force));
+        else return ksh::fail(KSH_OTHER_ERROR, "unknown arc-length method");
+
+        gsOptionList &op = alm->options();
+        op.setString("Solver", "SimplicialLDLT");
+        op.setInt("BifurcationMethod", o->bifurcation_method);
+        op.setReal("Length", o->length);
+        if (o->method == 0) {
+            op.setInt("AngleMethod", o->angle_method);
+            op.setReal("Scaling", o->scaling);
+        }
+        op.setSwitch("AdaptiveLength", o->adaptive != 0);
+        op.setInt("AdaptiveIterations", o->adaptive_iterations);
+        op.setReal("Tol", o->tol);
+        op.setReal("TolU", o->tol_u);
+        op.setReal("TolF", o->tol_f);
+        op.setInt("MaxIter", o->max_iter);
+        op.setSwitch("Verbose", o->verbose != 0);
+        op.setReal("Relaxation", o->relaxation);
+        if (o->quasi_iterations > 0) op.setInt("QuasiIterations", o->quasi_iterations);
+        op.setSwitch("Quasi", o->quasi != 0 || o->quasi_iterations > 0);
+        alm->applyOptions();
+
+        double t0 = now_s();
+        alm->initialize();
+        real_t Lold = 0;
+        gsVector<real_t> Uold = gsVector<real_t>::Zero(n);
+        alm->setIndicator(0.0);
+        bool bisected = false;
+        real_t dL = o->length, dL0 = o->length;
+        res->steps_done = 0;
+        res->halvings = 0;
+        res->final_status = 0;
+        for (int k = 0; k < o->nsteps; ++k) {
+            gsStatus status = alm->step();
+            res->final_status = (int)status;
+            if (status == gsStatus::NotConverged || status == gsStatus::AssemblyError) {
+                if (res->halvings >= o->max_halvings) break;
+                ++res->halvings;
+                dL = dL / 2.;
+                alm->setLength(dL);
+                alm->setSolution(Uold, Lold);
+                bisected = true;
+                k -= 1;
+                continue;
+            }
+            if (status != gsStatus::Success) break;
+            Uold = alm->solutionU();
+            Lold = alm->solutionL();
+            if (res->lambda) res->lambda[k] = Lold;
+            if (res->unorm) res->unorm[k] = Uold.norm();
+            if (res->indicator) res->indicator[k] = alm->indicator();
+            if (res->negatives) res->negatives[k] = alm->negatives();
+            if (res->iterations) res->iterations[k] = alm->numIterations();
+            if (res->residue_f) res->residue_f[k] = alm->residueF();
+            if (res->length) res->length[k] = dL;
+            if (res->U) std::memcpy(res->U + (size_t)k * n, Uold.data(), sizeof(double) * n);
+            if (res->w && res->probe_dof >= 0 && res->probe_dof < n) res->w[k] = Uold[res->probe_dof];
+            res->steps_done = k + 1;
+            if (!bisected) {
+                dL = dL0;
+                alm->setLength(dL);
+            }
+            bisected = false;
+        }
+        ksh_synchronize(pb);
+        res->wall_seconds = now_s() - t0;
+        if (auto *c = dynamic_cast<gsALMCrisfield<real_t> *>(alm.get())) res->phi = c->phi();
+        res->jacobians = alm->jacobianCount();
+        res->residuals = alm->residualCount();
+        res->solves = alm->solveCount();
+    } catch (const std::exception &e) {
+        return ksh::fail(KSH_OTHER_ERROR, e.what());
+    }
+    return res->steps_done == o->nsteps ? KSH_SUCCESS : KSH_NOT_CONVERGED;
+}
+
+extern "C" int ksh_run_static(ksh_problem *pb, const ksh_static_options *o, ksh_static_result *res)
+{
+    if (!pb || !o || !res) return ksh::fail(KSH_OTHER_ERROR, "null argument");
+    try {
+        gsSparseSolver<real_t>::settings().max_refine = o->max_refine;
+        gsSparseSolver<real_t>::settings().refine_tol = o->refine_tol;
+        gsShellOps ops(pb, o->precision);
+        const index_t n = ops.numDofs();
+        double t0 = now_s();
+        gsVector<real_t> F = ops.force();
+        auto Jacobian = ops.jacobian();
+        auto ALResidual = ops.staticALResidual();
+        gsSparseMatrix<real_t> K;
+        if (!Jacobian(gsVector<real_t>::Zero(n), K)) return ksh::fail(KSH_ASSEMBLY_ERROR, ksh_last_error());
+        gsStaticNewton<real_t> NWT(K, F, Jacobian, ALResidual);
+        gsOptionList &op = NWT.options();
+        op.setInt("maxIt", o->max_iter);
+        op.setReal("tol", o->tol);
+        op.setInt("verbose", o->verbose);
+        op.setString("Solver", "SimplicialLDLT");
+        op.setInt("BifurcationMethod", 0);
+        NWT.initialize();
+        gsControlDisplacement<real_t> control(&NWT);
+        res->steps_done = 0;
+        res->final_status = 0;
+        for (int k = 0; k < o->nsteps; ++k) {
+            gsStatus st = control.step(o->dlambda);
+            res->final_status = (int)st;
+            if (st != gsStatus::Success) break;
+            if (res->lambda) res->lambda[k] = control.solutionL();
+            if (res->unorm) res->unorm[k] = control.solutionU().norm();
+            if (res->iterations) res->iterations[k] = NWT.iterations() + 1;
+            if (res->negatives) res->negatives[k] = NWT.negatives();
+            if (res->residual) res->residual[k] = NWT.residualNorm();
+            if (res->U) std::memcpy(res->U + (size_t)k * n, control.solutionU().data(), sizeof(double) * n);
+            res->steps_done = k + 1;
+        }
+        ksh_synchronize(pb);
+        res->wall_seconds = now_s() - t0;
+        res->jacobians = NWT.jacobianCount();
+    } catch (const std::exception &e) {
+        return ksh::fail(KSH_OTHER_ERROR, e.what());
+    }
+    return res->steps_done == o->nsteps ? KSH_SUCCESS : KSH_NOT_CONVERGED;
+}

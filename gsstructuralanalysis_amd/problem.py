@@ -1,0 +1,201 @@
+"""Python front-end of libksh.so: one shell problem on the GPU, its band matrices and solvers.
+
+Mirrors the reference's operator surface (src/gsStructuralAnalysisTools/gsStructuralAnalysisTypes.h:58-93)
+for scripting and tests:
+
+* ``ShellProblem.residual(u, lam, conv)``  -- Residual_t / ALResidual_t bodies
+* ``ShellProblem.tangent(u, precision)``  -- Jacobian_t body (device-resident BandMatrix)
+* ``BandMatrix.factor / solve / vector_d / inertia`` -- gsSparseSolver<T>::SimplicialLDLT
+* ``ShellProblem.run_alm / run_static``   -- gsALMCrisfield / gsALMRiks / gsStaticNewton drivers
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .geometry import Patch as GeoPatch
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class BandMatrix:
+    """Device band matrix (lower band, ab[J*(bw+1)+(I-J)]) with its LDL^T factor."""
+
+    def __init__(self, problem: "ShellProblem", precision: int = L.FP64):
+        self.problem = problem
+        self.handle = C.c_void_p()
+        L.check(L.load().ksh_matrix_create(problem.handle, precision, C.byref(self.handle)))
+        self.n, self.bw, self.precision = problem.nfree, problem.bw, precision
+
+    def __del__(self):
+        try:
+            if self.handle:
+                L.load().ksh_matrix_destroy(self.handle)
+                self.handle = C.c_void_p()
+        except Exception:
+            pass
+
+    def download(self) -> np.ndarray:
+        ab = np.zeros(self.n * (self.bw + 1))
+        L.check(L.load().ksh_matrix_download(self.handle, _dptr(ab)))
+        return ab
+
+    def upload(self, ab):
+        ab = _f64(ab).ravel()
+        assert ab.size == self.n * (self.bw + 1)
+        L.check(L.load().ksh_matrix_upload(self.handle, _dptr(ab)))
+
+    def axpby(self, alpha, beta, other: "BandMatrix"):
+        L.check(L.load().ksh_matrix_axpby(self.handle, alpha, beta, other.handle))
+
+    def factor(self):
+        L.check(L.load().ksh_factor(self.handle))
+
+    def solve(self, b, max_refine=0, tol=1e-14):
+        b = _f64(b)
+        x = np.zeros(self.n)
+        L.check(L.load().ksh_solve(self.handle, _dptr(b), _dptr(x), max_refine, tol))
+        return x
+
+    def matvec(self, x):
+        x = _f64(x)
+        y = np.zeros(self.n)
+        L.check(L.load().ksh_matvec(self.handle, _dptr(x), _dptr(y)))
+        return y
+
+    def vector_d(self):
+        d = np.zeros(self.n)
+        L.check(L.load().ksh_vector_d(self.handle, _dptr(d)))
+        return d
+
+    def inertia(self):
+        neg = C.c_int64()
+        dmin = C.c_double()
+        L.check(L.load().ksh_inertia(self.handle, C.byref(neg), C.byref(dmin)))
+        return neg.value, dmin.value
+
+    def set_inertia_guard(self, guard):
+        L.check(L.load().ksh_set_inertia_guard(self.handle, guard))
+
+
+class ShellProblem:
+    """One single-patch Kirchhoff-Love shell problem resident on the GPU."""
+
+    def __init__(self, patch: GeoPatch):
+        lib = L.load()
+        self.patch = patch
+        self._ctrl = _f64(patch.ctrl).ravel()
+        self._fixed = np.ascontiguousarray(patch.fixed, dtype=np.int32)
+        self._pxi = _f64(patch.point_xi).ravel()
+        self._pF = _f64(patch.point_F).ravel()
+        pa = L.Patch(patch.p, patch.nx, patch.ny, _dptr(self._ctrl))
+        ma = L.Material(patch.t, patch.E, patch.nu, patch.rho)
+        bc = L.BC(self._fixed.ctypes.data_as(C.POINTER(C.c_int)))
+        lo = L.Loads(patch.point_xi.shape[0], _dptr(self._pxi), _dptr(self._pF), (C.c_double * 3)(*patch.fsurf))
+        self.handle = C.c_void_p()
+        L.check(lib.ksh_problem_create(C.byref(pa), C.byref(ma), C.byref(bc), C.byref(lo), C.byref(self.handle)))
+        n = C.c_int64()
+        bw = C.c_int64()
+        L.check(lib.ksh_num_free_dofs(self.handle, C.byref(n)))
+        L.check(lib.ksh_bandwidth(self.handle, C.byref(bw)))
+        self.nfree, self.bw = n.value, bw.value
+
+    def close(self):
+        if self.handle:
+            L.load().ksh_problem_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- operator bodies -------------------------------------------------------------------
+    def force(self):
+        F = np.zeros(self.nfree)
+        L.check(L.load().ksh_assemble_force(self.handle, _dptr(F)))
+        return F
+
+    def residual(self, u, lam=1.0, conv=L.RES_ALM):
+        u = _f64(u)
+        R = np.zeros(self.nfree)
+        L.check(L.load().ksh_assemble_residual(self.handle, _dptr(u), float(lam), conv, _dptr(R)))
+        return R
+
+    def tangent(self, u, precision=L.FP64, out: BandMatrix | None = None) -> BandMatrix:
+        K = out if out is not None else BandMatrix(self, precision)
+        u = _f64(u)
+        L.check(L.load().ksh_assemble_tangent(self.handle, _dptr(u), K.handle))
+        return K
+
+    def mass(self, precision=L.FP64) -> BandMatrix:
+        M = BandMatrix(self, precision)
+        L.check(L.load().ksh_assemble_mass(self.handle, M.handle))
+        return M
+
+    # -- drivers ---------------------------------------------------------------------------
+    def run_alm(self, nsteps, length, method=0, precision=L.FP64, max_iter=20, tol=1e-6, tol_u=1e-6, tol_f=1e-3,
+                scaling=-1.0, angle_method=0, bifurcation_method=0, relaxation=1.0, quasi=0, quasi_iterations=-1,
+                adaptive=0, adaptive_iterations=10, max_refine=0, refine_tol=1e-14, max_halvings=10, verbose=0,
+                keep_U=False, probe_dof=-1):
+        o = L.AlmOptions(method, precision, nsteps, length, max_iter, tol, tol_u, tol_f, scaling, angle_method,
+                         bifurcation_method, relaxation, quasi, quasi_iterations, adaptive, adaptive_iterations,
+                         max_refine, refine_tol, max_halvings, verbose)
+        out = {k: np.zeros(nsteps) for k in ("lambda", "unorm", "indicator", "residue_f", "length", "w")}
+        out["negatives"] = np.zeros(nsteps, dtype=np.int64)
+        out["iterations"] = np.zeros(nsteps, dtype=np.int32)
+        U = np.zeros(nsteps * self.nfree) if keep_U else None
+        r = L.AlmResult()
+        r.lambda_ = _dptr(out["lambda"])
+        r.unorm = _dptr(out["unorm"])
+        r.indicator = _dptr(out["indicator"])
+        r.negatives = out["negatives"].ctypes.data_as(C.POINTER(C.c_int64))
+        r.iterations = out["iterations"].ctypes.data_as(C.POINTER(C.c_int))
+        r.residue_f = _dptr(out["residue_f"])
+        r.length = _dptr(out["length"])
+        r.w = _dptr(out["w"])
+        r.probe_dof = probe_dof
+        r.U = _dptr(U) if U is not None else None
+        code = L.load().ksh_run_alm(self.handle, C.byref(o), C.byref(r))
+        if code not in (L.SUCCESS, L.NOT_CONVERGED):
+            L.check(code)
+        k = r.steps_done
+        res = {key: val[:k] for key, val in out.items()}
+        res.update(steps_done=k, halvings=r.halvings, final_status=r.final_status, wall_seconds=r.wall_seconds,
+                   phi=r.phi, jacobians=r.jacobians, residuals=r.residuals, solves=r.solves)
+        if U is not None:
+            res["U"] = U.reshape(nsteps, self.nfree)[:k]
+        return res
+
+    def run_static(self, nsteps, dlambda, precision=L.FP64, max_iter=25, tol=1e-10, max_refine=0,
+                   refine_tol=1e-14, verbose=0, keep_U=False):
+        o = L.StaticOptions(precision, nsteps, dlambda, max_iter, tol, max_refine, refine_tol, verbose)
+        out = {k: np.zeros(nsteps) for k in ("lambda", "unorm", "residual")}
+        out["iterations"] = np.zeros(nsteps, dtype=np.int32)
+        out["negatives"] = np.zeros(nsteps, dtype=np.int64)
+        U = np.zeros(nsteps * self.nfree) if keep_U else None
+        r = L.StaticResult()
+        r.lambda_ = _dptr(out["lambda"])
+        r.unorm = _dptr(out["unorm"])
+        r.residual = _dptr(out["residual"])
+        r.iterations = out["iterations"].ctypes.data_as(C.POINTER(C.c_int))
+        r.negatives = out["negatives"].ctypes.data_as(C.POINTER(C.c_int64))
+        r.U = _dptr(U) if U is not None else None
+        code = L.load().ksh_run_static(self.handle, C.byref(o), C.byref(r))
+        if code not in (L.SUCCESS, L.NOT_CONVERGED):
+            L.check(code)
+        k = r.steps_done
+        res = {key: val[:k] for key, val in out.items()}
+        res.update(steps_done=k, final_status=r.final_status, wall_seconds=r.wall_seconds, jacobians=r.jacobians)
+        if U is not None:
+            res["U"] = U.reshape(nsteps, self.nfree)[:k]
+        return res

@@ -1,0 +1,212 @@
+/*
+ * ksh.h -- C-ABI of the MI355X Kirchhoff-Love shell equilibrium path (libksh.so).
+ *
+ * Plain C: pointers and sizes only.  Every call returns an int status whose
+ * values are the ordinals of the reference's gsStatus
+ * (src/gsStructuralAnalysisTools/gsStructuralAnalysisTypes.h:22-30):
+ *   0 Success, 1 NotConverged, 2 AssemblyError, 3 SolverError, 4 NotStarted,
+ *   5 OtherError.  ksh_last_error() returns a message for the calling thread.
+ *
+ * Vectors passed as `const double *` / `double *` are HOST arrays unless the
+ * name ends in _dev (device pointers on the problem's device).  All work of one
+ * problem is issued on that problem's HIP stream; handles are not thread-safe.
+ *
+ * Dof numbering: control point cp = i + mx*j (i along xi, j along eta),
+ * full dof g = 3*cp + c; free dofs are the non-fixed full dofs in increasing g.
+ * Matrices are the symmetric free-dof matrices stored as a lower band with
+ * half-bandwidth bw: ab[J*(bw+1) + (I-J)], 0 <= I-J <= bw.
+ */
+#ifndef KSH_H
+#define KSH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum ksh_status {
+    KSH_SUCCESS = 0,
+    KSH_NOT_CONVERGED = 1,
+    KSH_ASSEMBLY_ERROR = 2,
+    KSH_SOLVER_ERROR = 3,
+    KSH_NOT_STARTED = 4,
+    KSH_OTHER_ERROR = 5
+};
+
+/* residual sign conventions of the two operator typedefs */
+enum ksh_residual_conv {
+    KSH_RES_STATIC = 0, /* R = lambda F - F_int(u)   (Residual_t, gsStaticNewton)  */
+    KSH_RES_ALM = 1     /* r = F_int(u) - lambda F   (ALResidual_t, gsALMBase)     */
+};
+
+/* matrix / factor precision */
+enum ksh_precision {
+    KSH_FP64 = 0,     /* FP64 assembly, FP64 band LDL^T                         */
+    KSH_FP32_IR = 1   /* FP32 assembly and band LDL^T on MFMA, FP64 refinement   */
+};
+
+typedef struct ksh_problem ksh_problem;
+typedef struct ksh_matrix ksh_matrix;
+
+typedef struct {
+    int p;              /* degree in both directions (2..4)                    */
+    int nx, ny;         /* elements per direction, open uniform knots on [0,1] */
+    const double *ctrl; /* (nx+p)*(ny+p) x 3 undeformed control points         */
+} ksh_patch;
+
+typedef struct {
+    double t, E, nu, rho; /* thickness, Young's modulus, Poisson, density      */
+} ksh_material;           /* St. Venant-Kirchhoff, analytical thickness integr. */
+
+typedef struct {
+    const int *fixed;   /* 3*(nx+p)*(ny+p) flags, 1 = homogeneous Dirichlet dof */
+} ksh_bc;
+
+typedef struct {
+    int npoint;
+    const double *point_xi; /* 2*npoint parametric points                      */
+    const double *point_F;  /* 3*npoint load vectors                           */
+    double fsurf[3];        /* dead surface force per unit undeformed area      */
+} ksh_loads;
+
+const char *ksh_last_error(void);
+int ksh_device_count(int *count);
+
+/* ---- problem ------------------------------------------------------------- */
+/* Replaces the gsThinShellAssembler<3,real_t,true> construction of the drivers
+ * (benchmarks/benchmark_Roof.cpp:309-314): patch + material + BCs + loads. */
+int ksh_problem_create(const ksh_patch *patch, const ksh_material *mat, const ksh_bc *bc,
+                       const ksh_loads *loads, ksh_problem **out);
+int ksh_problem_destroy(ksh_problem *pb);
+int ksh_num_free_dofs(const ksh_problem *pb, int64_t *n);
+int ksh_bandwidth(const ksh_problem *pb, int64_t *bw);
+int ksh_synchronize(ksh_problem *pb);
+
+/* F = rhs() of assemble() at u = 0: external load on the free dofs
+ * (benchmarks/benchmark_Roof.cpp:319-321). */
+int ksh_assemble_force(ksh_problem *pb, double *F_out);
+
+/* Residual_t / ALResidual_t body: constructSolution + assembleVector + the
+ * driver's sign convention (benchmarks/benchmark_Roof.cpp:335-344,
+ * tutorials/nonlinear_shell_static.cpp:129-136,
+ * examples/example_DisplacementControl.cpp:245-254). */
+int ksh_assemble_residual(ksh_problem *pb, const double *u_free, double lambda, int conv,
+                          double *R_out);
+int ksh_assemble_residual_dev(ksh_problem *pb, const double *u_free_dev, double lambda, int conv,
+                              double *R_out_dev);
+
+/* ---- matrices -------------------------------------------------------------- */
+int ksh_matrix_create(ksh_problem *pb, int prec, ksh_matrix **out);
+int ksh_matrix_destroy(ksh_matrix *K);
+int ksh_matrix_dims(const ksh_matrix *K, int64_t *n, int64_t *bw, int *prec);
+/* Jacobian_t body: constructSolution + assembleMatrix + matrix()
+ * (benchmarks/benchmark_Roof.cpp:323-332); K stays device resident. */
+int ksh_assemble_tangent(ksh_problem *pb, const double *u_free, ksh_matrix *K);
+int ksh_assemble_tangent_dev(ksh_problem *pb, const double *u_free_dev, ksh_matrix *K);
+/* assembleMass() + matrix() (solvers/modal_shell_XML.cpp:235-236). */
+int ksh_assemble_mass(ksh_problem *pb, ksh_matrix *M);
+/* band <-> host (tests, Eigen-compat downloads); values in FP64 */
+int ksh_matrix_download(const ksh_matrix *K, double *ab_out);
+int ksh_matrix_upload(ksh_matrix *K, const double *ab_in);
+/* Y = alpha*Y + beta*X (A - sigma B: src/gsEigenSolvers/gsEigenProblemBase.hpp:32,105;
+ * B -= A: src/gsEigenSolvers/gsBucklingSolver.h:104,144) */
+int ksh_matrix_axpby(ksh_matrix *Y, double alpha, double beta, const ksh_matrix *X);
+int ksh_matrix_copy(ksh_matrix *dst, const ksh_matrix *src);
+/* y = K x (SpMV used at src/gsALMSolvers/gsALMBase.hpp:639,663,720-721 and
+ * src/gsALMSolvers/gsALMCrisfield.hpp:316) */
+int ksh_matvec(const ksh_matrix *K, const double *x, double *y);
+int ksh_matvec_dev(const ksh_matrix *K, const double *x_dev, double *y_dev);
+
+/* ---- LDL^T: replaces gsSparseSolver<T>::SimplicialLDLT ---------------------
+ * compute()  src/gsALMSolvers/gsALMBase.hpp:186-198
+ * solve()    src/gsALMSolvers/gsALMBase.hpp:200-211
+ * vectorD()  src/gsALMSolvers/gsALMBase.hpp:559-562, src/gsStaticSolvers/gsStaticBase.h:176-177
+ * The factor lives beside the matrix; precision is the matrix's precision. */
+int ksh_factor(ksh_matrix *K);
+int ksh_solve(ksh_matrix *K, const double *b, double *x, int max_refine, double tol);
+int ksh_solve_dev(ksh_matrix *K, const double *b_dev, double *x_dev, int max_refine, double tol);
+/* countNegatives(vectorD()) and min(D) (src/gsALMSolvers/gsALMHelper.h:30-42,
+ * src/gsALMSolvers/gsALMBase.hpp:612-614); in FP32_IR mode a near-zero pivot
+ * (|d| < guard * max|d|) triggers an FP64 refactorization before counting. */
+int ksh_inertia(ksh_matrix *K, int64_t *n_negative, double *d_min);
+int ksh_vector_d(ksh_matrix *K, double *d_out);
+int ksh_set_inertia_guard(ksh_matrix *K, double guard);
+
+/* ---- solver drivers ---------------------------------------------------------
+ * The reference's solver classes (gsALMCrisfield / gsALMRiks, gsStaticNewton +
+ * gsControlDisplacement) run on the std::function operator surface bound to this
+ * problem, inside the same driver loops the reference's programs use. */
+typedef struct {
+    int method;              /* 0 Crisfield (gsALMCrisfield), 1 Riks (gsALMRiks)         */
+    int precision;           /* KSH_FP64 | KSH_FP32_IR                                   */
+    int nsteps;              /* converged arc-length steps to take                        */
+    double length;           /* arc length ("Length")                                    */
+    int max_iter;            /* "MaxIter"                                                */
+    double tol, tol_u, tol_f;/* "Tol", "TolU", "TolF"                                    */
+    double scaling;          /* Crisfield "Scaling" (phi), -1 = automatic                */
+    int angle_method;        /* Crisfield "AngleMethod"                                  */
+    int bifurcation_method;  /* "BifurcationMethod": -1 nothing, 0 determinant            */
+    double relaxation;       /* "Relaxation"                                             */
+    int quasi, quasi_iterations;
+    int adaptive, adaptive_iterations;
+    int max_refine;          /* refinement sweeps per solve (FP32_IR: FP64 residuals)     */
+    double refine_tol;       /* stop refinement at ||b - K x|| <= refine_tol ||b||         */
+    int max_halvings;        /* driver: halve the arc length on failure, retry            */
+    int verbose;
+} ksh_alm_options;
+
+typedef struct {
+    int steps_done;
+    int halvings;
+    int final_status;        /* gsStatus ordinal of the last step                        */
+    double *lambda;          /* [nsteps] load factor after each converged step (optional) */
+    double *unorm;           /* [nsteps] ||U||                                            */
+    double *indicator;       /* [nsteps] min D of the counted tangent                     */
+    int64_t *negatives;      /* [nsteps] negative pivots of the counted tangent           */
+    int *iterations;         /* [nsteps] corrector iterations                             */
+    double *residue_f;       /* [nsteps] ||r|| / ||lambda F|| at convergence              */
+    double *length;          /* [nsteps] arc length of the step                           */
+    double *U;               /* [nsteps*n] converged displacements (optional)             */
+    double *w;               /* [nsteps] value of U at the dof `probe_dof` (optional)     */
+    int probe_dof;
+    double wall_seconds;
+    double phi;              /* Crisfield scaling in use at the end                       */
+    int64_t jacobians, residuals, solves;
+} ksh_alm_result;
+
+/* benchmarks/benchmark_Roof.cpp:320-512: options, initialize, step loop with halving. */
+int ksh_run_alm(ksh_problem *pb, const ksh_alm_options *opt, ksh_alm_result *res);
+
+typedef struct {
+    int precision;
+    int nsteps;              /* load steps of dlambda (gsControlDisplacement::step)       */
+    double dlambda;
+    int max_iter;            /* "maxIt"                                                  */
+    double tol;              /* "tol"                                                    */
+    int max_refine;
+    double refine_tol;
+    int verbose;
+} ksh_static_options;
+
+typedef struct {
+    int steps_done;
+    int final_status;
+    double *lambda;          /* [nsteps]                                                 */
+    double *unorm;           /* [nsteps]                                                 */
+    int *iterations;         /* [nsteps]                                                 */
+    int64_t *negatives;      /* [nsteps]                                                 */
+    double *residual;        /* [nsteps] final ||R||                                      */
+    double *U;               /* [nsteps*n] (optional)                                     */
+    double wall_seconds;
+    int64_t jacobians;
+} ksh_static_result;
+
+/* examples/example_DisplacementControl.cpp:236-304 with a load-controlled residual:
+ * K_L, F from assemble(); gsStaticNewton(K_L, F, Jacobian, ALResidual); gsControlDisplacement. */
+int ksh_run_static(ksh_problem *pb, const ksh_static_options *opt, ksh_static_result *res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

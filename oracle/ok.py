@@ -1,0 +1,207 @@
+"""ctypes front-end of the FP64 CPU oracle (``oracle/_build/libksh_oracle.so``).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()`` as
+the checker and by ``bench.py``'s cpu_baseline leg.  The product never imports it.
+See ``ksh_oracle.h`` for the reference call sites each function restates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libksh_oracle.so")
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+class _Problem(C.Structure):
+    _fields_ = [("p", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("mx", C.c_int), ("my", C.c_int),
+                ("ctrl", C.POINTER(C.c_double)), ("t", C.c_double), ("E", C.c_double),
+                ("nu", C.c_double), ("rho", C.c_double), ("fixed", C.POINTER(C.c_int)),
+                ("npl", C.c_int), ("pl_xi", C.POINTER(C.c_double)), ("pl_F", C.POINTER(C.c_double)),
+                ("fsurf", C.c_double * 3)]
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        P = C.POINTER(_Problem)
+        L.ok_knots_uniform.argtypes = [C.c_int, C.c_int, _dp]
+        L.ok_find_span.argtypes = [C.c_int, C.c_int, C.c_double]
+        L.ok_ders_basis.argtypes = [C.c_int, _dp, C.c_int, C.c_double, C.c_int, _dp]
+        L.ok_gauss_legendre.argtypes = [C.c_int, _dp, _dp]
+        L.ok_free_map.argtypes = [P, _ip]
+        L.ok_bandwidth.argtypes = [P, _ip]
+        L.ok_energy.argtypes = [P, _dp]
+        L.ok_energy.restype = C.c_double
+        L.ok_internal_force.argtypes = [P, _dp, _dp]
+        L.ok_external_force.argtypes = [P, _dp]
+        L.ok_tangent_dense.argtypes = [P, _dp, _dp]
+        L.ok_mass_dense.argtypes = [P, _dp]
+        L.ok_tangent_band.argtypes = [P, _ip, C.c_int, C.c_int, _dp, _dp]
+        L.ok_mass_band.argtypes = [P, _ip, C.c_int, C.c_int, _dp]
+        L.ok_band_ldlt.argtypes = [C.c_int, C.c_int, _dp]
+        L.ok_band_solve.argtypes = [C.c_int, C.c_int, _dp, _dp]
+        L.ok_band_negatives.argtypes = [C.c_int, C.c_int, _dp, C.POINTER(C.c_double)]
+        L.ok_band_symv.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+# ---------------------------------------------------------------------------
+def ders_basis(p, U, span, x, nd):
+    out = np.zeros((nd + 1) * (p + 1))
+    lib().ok_ders_basis(p, np.ascontiguousarray(U, dtype=np.float64), span, x, nd, out)
+    return out.reshape(nd + 1, p + 1)
+
+
+def gauss_legendre(nq):
+    x, w = np.zeros(nq), np.zeros(nq)
+    lib().ok_gauss_legendre(nq, x, w)
+    return x, w
+
+
+class Oracle:
+    """FP64 CPU oracle bound to one Patch (see gsstructuralanalysis_amd.geometry.Patch)."""
+
+    def __init__(self, patch):
+        self.patch = patch
+        self._keep = [np.ascontiguousarray(patch.ctrl, dtype=np.float64).ravel(),
+                      np.ascontiguousarray(patch.fixed, dtype=np.int32),
+                      np.ascontiguousarray(patch.point_xi, dtype=np.float64).ravel(),
+                      np.ascontiguousarray(patch.point_F, dtype=np.float64).ravel()]
+        pb = _Problem()
+        pb.p, pb.nx, pb.ny, pb.mx, pb.my = patch.p, patch.nx, patch.ny, patch.mx, patch.my
+        pb.ctrl = self._keep[0].ctypes.data_as(C.POINTER(C.c_double))
+        pb.t, pb.E, pb.nu, pb.rho = patch.t, patch.E, patch.nu, patch.rho
+        pb.fixed = self._keep[1].ctypes.data_as(C.POINTER(C.c_int))
+        pb.npl = patch.point_xi.shape[0]
+        pb.pl_xi = self._keep[2].ctypes.data_as(C.POINTER(C.c_double))
+        pb.pl_F = self._keep[3].ctypes.data_as(C.POINTER(C.c_double))
+        pb.fsurf = (C.c_double * 3)(*patch.fsurf)
+        self.pb = pb
+        self.ndof = patch.ndof
+        self.free_idx = np.zeros(self.ndof, dtype=np.int32)
+        self.nfree = lib().ok_free_map(C.byref(pb), self.free_idx)
+        self.bw = lib().ok_bandwidth(C.byref(pb), self.free_idx)
+        self.free = np.flatnonzero(self.free_idx >= 0)
+
+    # full-dof quantities -------------------------------------------------------
+    def energy(self, u_full):
+        return lib().ok_energy(C.byref(self.pb), np.ascontiguousarray(u_full, dtype=np.float64))
+
+    def internal_force(self, u_full):
+        f = np.zeros(self.ndof)
+        lib().ok_internal_force(C.byref(self.pb), np.ascontiguousarray(u_full, dtype=np.float64), f)
+        return f
+
+    def external_force(self):
+        f = np.zeros(self.ndof)
+        lib().ok_external_force(C.byref(self.pb), f)
+        return f
+
+    def tangent_dense(self, u_full):
+        K = np.zeros((self.ndof, self.ndof))
+        lib().ok_tangent_dense(C.byref(self.pb), np.ascontiguousarray(u_full, dtype=np.float64), K)
+        return K
+
+    def mass_dense(self):
+        M = np.zeros((self.ndof, self.ndof))
+        lib().ok_mass_dense(C.byref(self.pb), M)
+        return M
+
+    # free-dof quantities (the solver-facing surface) --------------------------
+    def to_full(self, u_free):
+        u = np.zeros(self.ndof)
+        u[self.free] = u_free
+        return u
+
+    def force(self):
+        """F = assemble() rhs at zero displacement: the external load vector on free dofs."""
+        return self.external_force()[self.free]
+
+    def residual_alm(self, u_free, lam):
+        """ALResidual: r = F_int(u) - lam F  (benchmark_Roof.cpp:335-344)."""
+        return self.internal_force(self.to_full(u_free))[self.free] - lam * self.force()
+
+    def residual_static(self, u_free, lam=1.0):
+        """Residual: R = lam F - F_int(u)  (nonlinear_shell_static.cpp:129-136)."""
+        return -self.residual_alm(u_free, lam)
+
+    def tangent_band(self, u_free):
+        ab = np.zeros(self.nfree * (self.bw + 1))
+        lib().ok_tangent_band(C.byref(self.pb), self.free_idx, self.nfree, self.bw,
+                              np.ascontiguousarray(self.to_full(u_free)), ab)
+        return ab
+
+    def mass_band(self):
+        ab = np.zeros(self.nfree * (self.bw + 1))
+        lib().ok_mass_band(C.byref(self.pb), self.free_idx, self.nfree, self.bw, ab)
+        return ab
+
+    def tangent_free_dense(self, u_free):
+        return band_to_dense(self.tangent_band(u_free), self.nfree, self.bw)
+
+
+# ---------------------------------------------------------------------------
+# band helpers
+# ---------------------------------------------------------------------------
+def band_to_dense(ab, n, bw):
+    A = np.zeros((n, n))
+    ab = np.asarray(ab).reshape(n, bw + 1)
+    for d in range(bw + 1):
+        j = np.arange(n - d)
+        A[j + d, j] = ab[j, d]
+        A[j, j + d] = ab[j, d]
+    return A
+
+
+def dense_to_band(A, bw):
+    n = A.shape[0]
+    ab = np.zeros((n, bw + 1))
+    for d in range(bw + 1):
+        j = np.arange(n - d)
+        ab[j, d] = A[j + d, j]
+    return ab.ravel()
+
+
+class BandLDLT:
+    """Oracle banded LDL^T: compute / solve / vectorD / negatives (Eigen SimplicialLDLT semantics)."""
+
+    def __init__(self, ab, n, bw):
+        self.n, self.bw = n, bw
+        self.ab = np.array(ab, dtype=np.float64, copy=True)
+        self.info = lib().ok_band_ldlt(n, bw, self.ab)
+
+    def solve(self, b):
+        x = np.array(b, dtype=np.float64, copy=True)
+        lib().ok_band_solve(self.n, self.bw, self.ab, x)
+        return x
+
+    def vectorD(self):
+        return self.ab.reshape(self.n, self.bw + 1)[:, 0].copy()
+
+    def negatives(self):
+        dmin = C.c_double()
+        k = lib().ok_band_negatives(self.n, self.bw, self.ab, C.byref(dmin))
+        return k, dmin.value
+
+
+def band_symv(ab, n, bw, x):
+    y = np.zeros(n)
+    lib().ok_band_symv(n, bw, np.ascontiguousarray(ab, dtype=np.float64),
+                       np.ascontiguousarray(x, dtype=np.float64), y)
+    return y

@@ -1,0 +1,363 @@
+"""CPU restatement of the reference's solver loops over the FP64 oracle (TEST INFRASTRUCTURE ONLY).
+
+Used by tests/ as the checker of the GPU drivers and by bench.py's cpu_baseline leg.  Each
+method follows the reference file:line it cites; the linear algebra is the oracle's banded
+LDL^T (ok_band_ldlt), the assembly is the oracle's element loop.
+
+The ``faithful`` switch reproduces the reference's redundant work: the same tangent is factored
+again at gsALMCrisfield.hpp:69/114 and in the determinant stability branch at gsALMBase.hpp:561.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from .ok import BandLDLT, Oracle, band_symv
+
+
+class OracleOps:
+    """Jacobian / residual / force of a Patch on the CPU oracle, plus call counters."""
+
+    def __init__(self, patch):
+        self.o = Oracle(patch)
+        self.n, self.bw = self.o.nfree, self.o.bw
+        self.F = self.o.force()
+        self.n_jac = self.n_res = self.n_fact = self.n_solve = 0
+
+    def jacobian(self, u):
+        self.n_jac += 1
+        return self.o.tangent_band(u)
+
+    def residual_alm(self, u, lam):
+        self.n_res += 1
+        return self.o.residual_alm(u, lam)
+
+    def residual_static(self, u, lam):
+        self.n_res += 1
+        return self.o.residual_static(u, lam)
+
+    def factor(self, ab):
+        self.n_fact += 1
+        f = BandLDLT(ab, self.n, self.bw)
+        if f.info != 0:
+            raise _Err(3)
+        return f
+
+    def solve(self, fac, b):
+        self.n_solve += 1
+        return fac.solve(b)
+
+
+class _Err(Exception):
+    def __init__(self, code):
+        super().__init__(code)
+        self.code = code
+
+
+# ---------------------------------------------------------------------------------------------
+class ALMCrisfield:
+    """gsALMBase::_step (gsALMBase.hpp:353-416) with gsALMCrisfield's methods (gsALMCrisfield.hpp:40-472)."""
+
+    def __init__(self, ops: OracleOps, length, max_iter=100, tol_f=1e-3, tol_u=1e-6, scaling=-1.0, angle_method=0,
+                 relaxation=1.0, bifurcation_method=0, faithful=False):
+        self.ops = ops
+        self.F = ops.F
+        n = ops.n
+        self.U = np.zeros(n)
+        self.DeltaU = np.zeros(n)
+        self.L = self.DeltaL = 0.0
+        self.DeltaUold = np.zeros(n)
+        self.DeltaLold = 0.0
+        self.Uguess = None
+        self.arc = length
+        self.max_iter, self.tol_f, self.tol_u = max_iter, tol_f, tol_u
+        self.phi = scaling
+        self.phi_user = scaling != -1
+        self.angle = angle_method
+        self.relax = relaxation
+        self.bif = bifurcation_method
+        self.faithful = faithful
+        self.indicator, self.negatives = 0.0, 0
+        self.stability = self.stability_prev = -1
+        self.deltaUt = np.zeros(n)
+        self.deltaUbar = np.zeros(n)
+        self.deltaU = np.zeros(n)
+        self.deltaL = 0.0
+        self.eta = 1.0
+        self.jac = None
+        self.fac = None
+        self.num_iter = 0
+
+    # gsALMBase::init -> _computeStability(m_U) (gsALMBase.hpp:95-103)
+    def initialize(self):
+        self._compute_jacobian(self.U)
+        self._stability()
+        self.stability = -1 if self.indicator >= 0 else 1
+
+    def _compute_jacobian(self, U):
+        self.jac = self.ops.jacobian(U)
+        self.fac = self.ops.factor(self.jac)
+
+    def _stability(self):
+        """Determinant branch of _computeStability (gsALMBase.hpp:557-569,612-614)."""
+        if self.bif < 0:
+            self.negatives, self.indicator = 0, 0.0
+        else:
+            if self.faithful:
+                self.fac = self.ops.factor(self.jac)
+            self.negatives, self.indicator = self.fac.negatives()
+        self.stability = 1 if self.indicator < 0 else -1
+
+    def _residual(self):
+        self.resVec = self.ops.residual_alm(self.U + self.DeltaU, self.L + self.DeltaL)
+
+    def _norms(self):
+        """computeResidualNorms (gsALMBase.hpp:150-183)."""
+        if self.num_iter == 0:
+            self.basisF = np.linalg.norm((self.L + self.DeltaL) * self.F)
+            self.basisU = np.linalg.norm(self.DeltaU)
+            self.residueF = np.linalg.norm(self.resVec) / self.basisF
+            self.residueU = np.linalg.norm(self.deltaU) / np.linalg.norm(self.DeltaU)
+        else:
+            self.residueF = np.linalg.norm(self.resVec) / self.basisF
+            self.residueU = np.linalg.norm(self.deltaU) / self.basisU
+
+    # gsALMCrisfield::predictor (gsALMCrisfield.hpp:110-164)
+    def predictor(self):
+        self._compute_jacobian(self.U + self.DeltaU)
+        if self.faithful:
+            self.fac = self.ops.factor(self.jac)
+        self.deltaUt = self.ops.solve(self.fac, self.F)
+        FF = self.F @ self.F
+        if self.DeltaUold @ self.DeltaUold == 0 and self.DeltaLold * self.DeltaLold == 0:
+            self.deltaL = self.arc / np.sqrt(2 * (self.deltaUt @ self.deltaUt))
+            self.deltaU = self.deltaUbar + self.deltaL * self.deltaUt
+            if not self.phi_user:
+                self.phi = np.sqrt((self.deltaUt @ self.deltaUt) / FF)
+        else:
+            if not self.phi_user:
+                self.phi = np.sqrt((self.U @ self.U) / (self.L ** 2 * FF))
+            self._lambda_mu()
+        self.DeltaU = self.DeltaU + self.deltaU
+        self.DeltaL += self.deltaL
+        if self.angle in (1, 2):
+            self.DeltaUold = self.DeltaU.copy()
+            self.DeltaLold = self.DeltaL
+
+    def _lambda_mu(self):
+        """computeLambdaMU (gsALMCrisfield.hpp:383-398)."""
+        A0 = self.phi ** 2 * (self.F @ self.F)
+        dir_ = np.sign(self.DeltaUold @ self.deltaUt + A0 * self.DeltaLold)
+        denum = np.sqrt(self.deltaUt @ self.deltaUt + A0)
+        mu = self.arc if denum == 0 else self.arc / denum
+        self.deltaL = dir_ * mu
+        self.deltaU = self.deltaL * self.deltaUt
+
+    def _lambdas(self):
+        """computeLambdas + Simple/Modified/Eta/Complex/DOT (gsALMCrisfield.hpp:212-363,400-472)."""
+        F = self.F
+        A0 = self.phi ** 2 * (F @ F)
+        ut, ub, DU = self.deltaUt, self.deltaUbar, self.DeltaU
+        a0 = ut @ ut + A0
+        b0 = 2 * (ut @ DU + self.DeltaL * A0)
+        b1 = 2 * (ub @ ut)
+        c0 = DU @ DU + self.DeltaL ** 2 * A0 - self.arc ** 2
+        c1 = 2 * (DU @ ub)
+        c2 = ub @ ub
+        eta = self.eta
+        al1, al2, al3 = a0, b0 + eta * b1, c0 + eta * c1 + eta * eta * c2
+        disc = al2 ** 2 - 4 * al1 * al3
+        if disc >= 0:
+            self.eta = 1.0
+            lams = [(-al2 + np.sqrt(disc)) / (2 * al1), (-al2 - np.sqrt(disc)) / (2 * al1)]
+            self._dot(lams)
+            return
+        # modified (Lam 1992)
+        m1 = b1 * b1 - 4.0 * a0 * c2
+        m2 = 2.0 * b0 * b1 - 4.0 * a0 * c1
+        m3 = b0 * b0 - 4.0 * a0 * c0
+        disc2 = m2 ** 2 - 4.0 * m1 * m3
+        if disc2 >= 0:
+            e0 = (-m2 + np.sqrt(disc2)) / (2.0 * m1)
+            e1 = (-m2 - np.sqrt(disc2)) / (2.0 * m1)
+            eta1, eta2 = min(e0, e1), max(e0, e1)
+            xi = 0.05 * abs(eta2 - eta1)
+            if eta2 < 1.0:
+                self.eta = eta2 - xi
+            elif eta2 > 1.0 and -m2 / m1 < 1.0:
+                self.eta = eta2 + xi
+            elif eta1 < 1.0 and -m2 / m1 > 1.0:
+                self.eta = eta1 - xi
+            elif eta1 > 1.0:
+                self.eta = eta1 + xi
+        if disc2 >= 0 and self.eta > 0.05:
+            al2 = b0 + self.eta * b1
+            lam = -al2 / (2 * a0)
+            self._dot([lam, lam])
+        else:
+            self.eta = 1.0
+            DeltaUcr = self.DeltaU + self.deltaUbar
+            Fint = band_symv(self.jac, self.ops.n, self.ops.bw, self.U + self.DeltaU)
+            Lcr = (Fint @ F) / (F @ F)
+            DeltaLcr = Lcr - self.L
+            arc_cr = np.sqrt(DeltaUcr @ DeltaUcr + A0 * DeltaLcr ** 2)
+            mu = self.arc / arc_cr
+            self.deltaL = mu * DeltaLcr - self.DeltaL
+            self.deltaU = mu * DeltaUcr - self.DeltaU
+
+    def _dot(self, lams):
+        base = self.DeltaUold @ self.deltaUt + self.phi ** 2 * self.DeltaLold
+        d1, d2 = lams[0] * base, lams[1] * base
+        pick = 1 if d1 < d2 else 0
+        self.deltaL = lams[pick]
+        self.deltaU = self.eta * self.deltaUbar + self.deltaUt * lams[pick]
+
+    # gsALMCrisfield::iteration / quasiNewtonIteration (gsALMCrisfield.hpp:65-100)
+    def iteration(self):
+        self._compute_jacobian(self.U + self.DeltaU)
+        if self.faithful:
+            self.fac = self.ops.factor(self.jac)
+        self.deltaUt = self.ops.solve(self.fac, self.F)
+        self.deltaUbar = self.ops.solve(self.fac, -self.resVec)
+        self.eta = 1.0
+        lamold = self.deltaL
+        self._lambdas()
+        if (lamold * self.deltaL < 0 and abs(self.deltaL) <= abs(lamold)) and self.relax != 1.0:
+            self.deltaU = self.relax * (self.deltaL * self.deltaUt + self.eta * self.deltaUbar)
+            self.deltaL = self.relax * self.deltaL
+        self.DeltaU = self.DeltaU + self.deltaU
+        self.DeltaL += self.deltaL
+        if self.angle == 1:
+            self.DeltaUold = self.DeltaU.copy()
+            self.DeltaLold = self.DeltaL
+
+    def step(self):
+        """gsALMBase::step/_step (gsALMBase.hpp:327-416); returns 0 success, 1 not converged, 3 solver error."""
+        try:
+            self._step()
+            return 0
+        except _Err as e:
+            return e.code
+
+    def _step(self):
+        self.num_iter = 0
+        self.DeltaU = np.zeros(self.ops.n)
+        self.deltaUbar = np.zeros(self.ops.n)
+        self.deltaUt = np.zeros(self.ops.n)
+        self.deltaU = np.zeros(self.ops.n)
+        self.DeltaL = self.deltaL = 0.0
+        self.eta = 1.0
+        self.predictor()
+        self._residual()
+        self._norms()
+        self.stability_prev = self.stability
+        for it in range(1, self.max_iter):
+            self.num_iter = it
+            self.iteration()
+            self._stability()
+            self._residual()
+            self._norms()
+            if self.residueF < self.tol_f and self.residueU < self.tol_u:
+                self.U = self.U + self.DeltaU
+                self.L += self.DeltaL
+                if self.angle == 0:
+                    self.DeltaUold = self.DeltaU.copy()
+                    self.DeltaLold = self.DeltaL
+                return
+            if it == self.max_iter - 1:
+                raise _Err(1)
+
+    def set_solution(self, U, L):
+        self.U, self.L = U.copy(), L
+
+
+def run_crisfield(patch, nsteps, length, max_halvings=10, **kw):
+    """Driver loop of benchmarks/benchmark_Roof.cpp:417-512 on the oracle."""
+    ops = OracleOps(patch)
+    alm = ALMCrisfield(ops, length, **kw)
+    t0 = time.perf_counter()
+    alm.initialize()
+    Uold, Lold = np.zeros(ops.n), 0.0
+    dL0 = dL = length
+    bisected = False
+    out = {"lambda": [], "unorm": [], "negatives": [], "indicator": [], "iterations": [], "length": [], "U": [],
+           "residue_f": []}
+    halvings = 0
+    k = 0
+    while k < nsteps:
+        st = alm.step()
+        if st == 1:
+            if halvings >= max_halvings:
+                break
+            halvings += 1
+            dL /= 2.0
+            alm.arc = dL
+            alm.set_solution(Uold, Lold)
+            bisected = True
+            continue
+        if st != 0:
+            break
+        Uold, Lold = alm.U.copy(), alm.L
+        out["lambda"].append(Lold)
+        out["unorm"].append(np.linalg.norm(Uold))
+        out["negatives"].append(alm.negatives)
+        out["indicator"].append(alm.indicator)
+        out["iterations"].append(alm.num_iter)
+        out["length"].append(dL)
+        out["U"].append(Uold)
+        out["residue_f"].append(alm.residueF)
+        if not bisected:
+            dL = dL0
+            alm.arc = dL
+        bisected = False
+        k += 1
+    res = {key: np.array(v) for key, v in out.items()}
+    res.update(steps_done=len(out["lambda"]), halvings=halvings, wall_seconds=time.perf_counter() - t0,
+               phi=alm.phi, jacobians=ops.n_jac, factorizations=ops.n_fact, solves=ops.n_solve,
+               residuals=ops.n_res)
+    return res
+
+
+# ---------------------------------------------------------------------------------------------
+def run_static_newton(patch, nsteps, dlambda, max_iter=25, tol=1e-10):
+    """gsStaticNewton::_solveNonlinear (gsStaticNewton.hpp:141-193,273-316) under
+    gsControlDisplacement::step (gsControlDisplacement.h:46-60), static residual lam F - F_int."""
+    ops = OracleOps(patch)
+    n = ops.n
+    t0 = time.perf_counter()
+    Uc = np.zeros(n)
+    Lc = 0.0
+    out = {"lambda": [], "unorm": [], "iterations": [], "negatives": [], "residual": [], "U": []}
+    for k in range(nsteps):
+        lam = Lc + dlambda
+        U = Uc.copy()
+        DeltaU = np.zeros(n)
+        R = ops.residual_static(U, lam)
+        res = np.linalg.norm(R)
+        res = 1.0 if res == 0 else res
+        res_ini = res
+        conv = False
+        for it in range(max_iter):
+            fac = ops.factor(ops.jacobian(U + DeltaU))
+            deltaU = ops.solve(fac, R)
+            DeltaU = DeltaU + deltaU
+            R = ops.residual_static(U + DeltaU, lam)
+            res = np.linalg.norm(R)
+            if np.linalg.norm(deltaU) / np.linalg.norm(DeltaU) < tol and res / res_ini < tol:
+                conv = True
+                break
+        if not conv:
+            break
+        neg, _ = fac.negatives()
+        Lc = lam
+        Uc = Uc + DeltaU
+        out["lambda"].append(Lc)
+        out["unorm"].append(np.linalg.norm(Uc))
+        out["iterations"].append(it + 1)
+        out["negatives"].append(neg)
+        out["residual"].append(res)
+        out["U"].append(Uc.copy())
+    r = {key: np.array(v) for key, v in out.items()}
+    r.update(steps_done=len(out["lambda"]), wall_seconds=time.perf_counter() - t0, jacobians=ops.n_jac)
+    return r
