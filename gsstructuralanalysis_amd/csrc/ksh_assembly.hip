@@ -25,11 +25,9 @@
 #include <cmath>
 
 #include "ksh_internal.hpp"
+#include "ksh_mfma.hpp"
 
 namespace ksh {
-
-typedef double v4d __attribute__((ext_vector_type(4)));
-typedef float v4f __attribute__((ext_vector_type(4)));
 
 namespace {
 
@@ -301,15 +299,6 @@ __global__ __launch_bounds__(256) void k_residual_elem(ElemArgs ar, double *Fe)
     }
 }
 
-template <typename T> struct Mfma;
-template <> struct Mfma<double> {
-    typedef v4d acc_t;
-    __device__ static acc_t run(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
-};
-template <> struct Mfma<float> {
-    typedef v4f acc_t;
-    __device__ static acc_t run(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-};
 
 // Element tangent: per component row i, H rows of component i -> Z^i = H Psi -> 3 MFMA tiles.
 template <int P, typename T>
@@ -346,13 +335,13 @@ __global__ __launch_bounds__(256) void k_tangent_elem(ElemArgs ar, T *Ke)
         __syncthreads();
         if (wave < 3) {
             const int k = wave;
-            typename Mfma<T>::acc_t acc = {0, 0, 0, 0};
+            typename Mfma16<T>::acc_t acc = {0, 0, 0, 0};
             for (int st = 0; st < NKK / 4; ++st) {
                 int kk = 4 * st + lane / 16;
-                acc = Mfma<T>::run(sPsiT[kk][lane % 16], sZ[kk][k][lane % 16], acc);
+                acc = Mfma16<T>::run(sPsiT[kk][lane % 16], sZ[kk][k][lane % 16], acc);
             }
             T *tile = Kout + (i * 3 + k) * 256;
-            for (int v = 0; v < 4; ++v) tile[(4 * (lane / 16) + v) * 16 + lane % 16] = acc[v];
+            for (int v = 0; v < 4; ++v) tile[Mfma16<T>::row(lane, v) * 16 + lane % 16] = acc[v];
         }
         __syncthreads();
     }

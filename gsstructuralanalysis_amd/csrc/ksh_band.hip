@@ -23,23 +23,12 @@
 #include <cstring>
 
 #include "ksh_internal.hpp"
+#include "ksh_mfma.hpp"
 
 namespace ksh {
 
 namespace {
 
-typedef double v4d __attribute__((ext_vector_type(4)));
-typedef float v4f __attribute__((ext_vector_type(4)));
-
-template <typename T> struct BandMfma;
-template <> struct BandMfma<double> {
-    typedef v4d acc_t;
-    __device__ static acc_t run(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
-};
-template <> struct BandMfma<float> {
-    typedef v4f acc_t;
-    __device__ static acc_t run(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-};
 
 constexpr int NB = 16;
 constexpr int kThreads = 256;
@@ -138,11 +127,11 @@ __global__ __launch_bounds__(kThreads) void k_band_ldlt(T *fac, int n, int bw, i
             int tj = pidx - ti * (ti + 1) / 2;
             const int rj0 = w + tj * 16, ri0 = w + ti * 16;
             // accumulator rows = j (A's column), cols = i (A's row): contiguous stores
-            typename BandMfma<T>::acc_t acc;
+            typename Mfma16<T>::acc_t acc;
             const int ri = ri0 + lane % 16;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                int rj = rj0 + 4 * (lane / 16) + v;
+                int rj = rj0 + Mfma16<T>::row(lane, v);
                 T x = 0;
                 if (ri < nr && rj < nr && ri >= rj) x = fac[(size_t)(c0 + rj) * ld + (ri - rj)];
                 acc[v] = x;
@@ -152,11 +141,11 @@ __global__ __launch_bounds__(kThreads) void k_band_ldlt(T *fac, int n, int bw, i
                 int kk = 4 * s + lane / 16;
                 T wv = -pan[(size_t)kk * PR + rj0 + lane % 16] * dv[kk];
                 T lv = pan[(size_t)kk * PR + ri0 + lane % 16];
-                acc = BandMfma<T>::run(wv, lv, acc);
+                acc = Mfma16<T>::run(wv, lv, acc);
             }
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                int rj = rj0 + 4 * (lane / 16) + v;
+                int rj = rj0 + Mfma16<T>::row(lane, v);
                 if (ri < nr && rj < nr && ri >= rj) fac[(size_t)(c0 + rj) * ld + (ri - rj)] = acc[v];
             }
         }

@@ -272,10 +272,60 @@ class ALMCrisfield:
         self.U, self.L = U.copy(), L
 
 
-def run_crisfield(patch, nsteps, length, max_halvings=10, **kw):
-    """Driver loop of benchmarks/benchmark_Roof.cpp:417-512 on the oracle."""
+class ALMRiks(ALMCrisfield):
+    """gsALMRiks (gsALMRiks.hpp:22-134,205-212): linearised constraint with phi = 1/n."""
+
+    def __init__(self, ops, length, **kw):
+        super().__init__(ops, length, **kw)
+        self.Uprev = np.zeros(ops.n)
+        self.Lprev = 0.0
+        self.arc_prev = length
+
+    def predictor(self):
+        self._compute_jacobian(self.U + self.DeltaU)
+        if self.faithful:
+            self.fac = self.ops.factor(self.jac)
+        n = self.ops.n
+        self.phi = 0.99999999999 if n == 1 else 1.0 / n
+        if np.linalg.norm(self.U - self.Uprev) < 1e-10 and (self.L - self.Lprev) ** 2 < 1e-10:
+            DL = 1.0
+            self.deltaUt = self.ops.solve(self.fac, self.F)
+            den = np.sqrt(self.deltaUt @ self.deltaUt + self.DeltaL * DL)
+            self.deltaU = self.deltaUt / den
+            self.deltaL = DL / den
+        else:
+            self.deltaL = (self.L - self.Lprev) / self.arc_prev
+            self.deltaU = (self.U - self.Uprev) / self.arc_prev
+        self.deltaL *= self.arc
+        self.deltaU = self.deltaU * self.arc
+        self.DeltaU = self.DeltaU + self.deltaU
+        self.DeltaL += self.deltaL
+
+    def iteration(self):
+        self._compute_jacobian(self.U + self.DeltaU)
+        if self.faithful:
+            self.fac = self.ops.factor(self.jac)
+        self.deltaUt = self.ops.solve(self.fac, self.F)
+        self.deltaUbar = self.ops.solve(self.fac, -self.resVec)
+        DU = self.DeltaU
+        r = self.phi * (DU @ DU) + (1 - self.phi) * self.DeltaL ** 2 - self.arc ** 2
+        self.deltaL = -(r + 2 * self.phi * (DU @ self.deltaUbar)) / (
+            2 * (1 - self.phi) * self.DeltaL + 2 * self.phi * (DU @ self.deltaUt))
+        self.deltaU = self.deltaUbar + self.deltaL * self.deltaUt
+        self.DeltaU = self.DeltaU + self.deltaU
+        self.DeltaL += self.deltaL
+
+    def _step(self):
+        U0, L0 = self.U.copy(), self.L
+        super()._step()
+        self.Uprev, self.Lprev = U0, L0
+        self.arc_prev = self.arc
+
+
+def run_crisfield(patch, nsteps, length, max_halvings=10, method=0, **kw):
+    """Driver loop of benchmarks/benchmark_Roof.cpp:417-512 on the oracle (method 0 Crisfield, 1 Riks)."""
     ops = OracleOps(patch)
-    alm = ALMCrisfield(ops, length, **kw)
+    alm = (ALMCrisfield if method == 0 else ALMRiks)(ops, length, **kw)
     t0 = time.perf_counter()
     alm.initialize()
     Uold, Lold = np.zeros(ops.n), 0.0

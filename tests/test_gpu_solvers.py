@@ -42,9 +42,13 @@ def test_crisfield_auto_phi_and_riks(ksh):
     got = sp.run_alm(12, 1.5e-3, method=0, max_iter=20, tol_f=1e-8, tol_u=1e-6, scaling=-1.0)
     assert np.abs(got["lambda"] - ref["lambda"]).max() <= 1e-7 * np.abs(ref["lambda"]).max()
     assert abs(got["phi"] - ref["phi"]) <= 1e-7 * abs(ref["phi"])
+    print("[case] riks roof p=3 6x6", flush=True)
+    riks_ref = run_crisfield(pt, 8, 1.5e-3, method=1, max_iter=20, tol_f=1e-8, tol_u=1e-6)
     riks = sp.run_alm(8, 1.5e-3, method=1, max_iter=20, tol_f=1e-8, tol_u=1e-6)
-    assert riks["steps_done"] == 8
-    assert (riks["residue_f"] < 1e-8).all() and np.all(np.diff(riks["lambda"]) > 0)
+    assert riks["steps_done"] == riks_ref["steps_done"] == 8
+    assert (riks["residue_f"] < 1e-8).all()
+    assert np.abs(riks["lambda"] - riks_ref["lambda"]).max() <= 1e-7 * np.abs(riks_ref["lambda"]).max()
+    assert np.abs(riks["unorm"] - riks_ref["unorm"]).max() <= 1e-7 * np.abs(riks_ref["unorm"]).max()
 
 
 def test_crisfield_fp32_refined_matches_fp64_path(ksh):
