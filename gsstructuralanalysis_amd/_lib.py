@@ -93,6 +93,9 @@ SIGNATURES = {
     "ksh_inertia": (C.c_int, [_VP, _i64p, _dp]),
     "ksh_vector_d": (C.c_int, [_VP, _dp]),
     "ksh_set_inertia_guard": (C.c_int, [_VP, C.c_double]),
+    "ksh_profile_enable": (C.c_int, [C.c_int]),
+    "ksh_profile_reset": (C.c_int, []),
+    "ksh_profile_query": (C.c_int, [C.c_char_p, _i64p, _dp]),
     "ksh_run_alm": (C.c_int, [_VP, C.POINTER(AlmOptions), C.POINTER(AlmResult)]),
     "ksh_run_static": (C.c_int, [_VP, C.POINTER(StaticOptions), C.POINTER(StaticResult)]),
 }

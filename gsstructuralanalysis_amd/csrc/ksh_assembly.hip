@@ -26,6 +26,7 @@
 
 #include "ksh_internal.hpp"
 #include "ksh_mfma.hpp"
+#include "ksh_profile.hpp"
 
 namespace ksh {
 
@@ -567,12 +568,16 @@ int launch_construct_solution(Problem &pb, const double *u_free_dev)
 int launch_residual(Problem &pb, double lambda, int conv, double *R_dev)
 {
     ElemArgs ar = elem_args(pb);
-    if (pb.p == 2)
-        hipLaunchKernelGGL(k_residual_elem<2>, dim3(pb.nel), dim3(256), 0, pb.stream, ar, pb.Fe);
-    else
-        hipLaunchKernelGGL(k_residual_elem<3>, dim3(pb.nel), dim3(256), 0, pb.stream, ar, pb.Fe);
+    {
+        ProfScope ps("residual_elem", pb.stream);
+        if (pb.p == 2)
+            hipLaunchKernelGGL(k_residual_elem<2>, dim3(pb.nel), dim3(256), 0, pb.stream, ar, pb.Fe);
+        else
+            hipLaunchKernelGGL(k_residual_elem<3>, dim3(pb.nel), dim3(256), 0, pb.stream, ar, pb.Fe);
+    }
     KSH_HIP(hipGetLastError());
     int mode = (conv == KSH_RES_ALM) ? 1 : 2;
+    ProfScope ps("vector_gather", pb.stream);
     hipLaunchKernelGGL(k_vector_gather, dim3(nblk(pb.nfree, 256)), dim3(256), 0, pb.stream, pb.nfree, pb.p, pb.nx, pb.ny,
                        pb.mx, pb.free2full, pb.Fe, pb.F, lambda, mode, R_dev);
     KSH_HIP(hipGetLastError());
@@ -609,20 +614,28 @@ int launch_tangent(Problem &pb, Matrix &K)
     size_t slots = (size_t)K.n * K.ld;
     if (K.prec == KSH_FP64) {
         double *Ke = (double *)pb.Ke;
-        if (pb.p == 2)
-            hipLaunchKernelGGL((k_tangent_elem<2, double>), dim3(pb.nel), dim3(256), 0, pb.stream, ar, Ke);
-        else
-            hipLaunchKernelGGL((k_tangent_elem<3, double>), dim3(pb.nel), dim3(256), 0, pb.stream, ar, Ke);
+        {
+            ProfScope ps("tangent_elem", pb.stream);
+            if (pb.p == 2)
+                hipLaunchKernelGGL((k_tangent_elem<2, double>), dim3(pb.nel), dim3(256), 0, pb.stream, ar, Ke);
+            else
+                hipLaunchKernelGGL((k_tangent_elem<3, double>), dim3(pb.nel), dim3(256), 0, pb.stream, ar, Ke);
+        }
         KSH_HIP(hipGetLastError());
+        ProfScope ps("band_gather", pb.stream);
         hipLaunchKernelGGL(k_band_gather<double>, dim3(nblk(slots, 256)), dim3(256), 0, pb.stream, K.n, K.bw, pb.p, pb.nx,
                            pb.ny, pb.mx, pb.free2full, Ke, (double *)K.ab);
     } else {
         float *Ke = (float *)pb.Ke;
-        if (pb.p == 2)
-            hipLaunchKernelGGL((k_tangent_elem<2, float>), dim3(pb.nel), dim3(256), 0, pb.stream, ar, Ke);
-        else
-            hipLaunchKernelGGL((k_tangent_elem<3, float>), dim3(pb.nel), dim3(256), 0, pb.stream, ar, Ke);
+        {
+            ProfScope ps("tangent_elem", pb.stream);
+            if (pb.p == 2)
+                hipLaunchKernelGGL((k_tangent_elem<2, float>), dim3(pb.nel), dim3(256), 0, pb.stream, ar, Ke);
+            else
+                hipLaunchKernelGGL((k_tangent_elem<3, float>), dim3(pb.nel), dim3(256), 0, pb.stream, ar, Ke);
+        }
         KSH_HIP(hipGetLastError());
+        ProfScope ps("band_gather", pb.stream);
         hipLaunchKernelGGL(k_band_gather<float>, dim3(nblk(slots, 256)), dim3(256), 0, pb.stream, K.n, K.bw, pb.p, pb.nx,
                            pb.ny, pb.mx, pb.free2full, Ke, (float *)K.ab);
     }

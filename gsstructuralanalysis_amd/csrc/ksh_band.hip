@@ -24,6 +24,7 @@
 
 #include "ksh_internal.hpp"
 #include "ksh_mfma.hpp"
+#include "ksh_profile.hpp"
 
 namespace ksh {
 
@@ -341,9 +342,11 @@ int band_factor(Matrix &K, void *fac, int prec)
         return fail(KSH_SOLVER_ERROR, "band too wide for the one-workgroup factorization (LDS panel exceeds 160 KB)");
     if (prec == KSH_FP64) {
         KSH_HIP(hipFuncSetAttribute((const void *)k_band_ldlt<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+        ProfScope ps("band_ldlt", pb.stream);
         hipLaunchKernelGGL(k_band_ldlt<double>, dim3(1), dim3(kThreads), smem, pb.stream, (double *)fac, K.n, K.bw, K.flag);
     } else {
         KSH_HIP(hipFuncSetAttribute((const void *)k_band_ldlt<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+        ProfScope ps("band_ldlt", pb.stream);
         hipLaunchKernelGGL(k_band_ldlt<float>, dim3(1), dim3(kThreads), smem, pb.stream, (float *)fac, K.n, K.bw, K.flag);
     }
     KSH_HIP(hipGetLastError());
@@ -358,6 +361,7 @@ int band_factor(Matrix &K, void *fac, int prec)
 int band_solve(Matrix &K, const void *fac, int prec, double *x_dev)
 {
     Problem &pb = *K.pb;
+    ProfScope ps("band_solve", pb.stream);
     if (prec == KSH_FP64)
         hipLaunchKernelGGL(k_band_solve<double>, dim3(1), dim3(kThreads), 0, pb.stream, (const double *)fac, K.n, K.bw, x_dev);
     else
@@ -369,6 +373,7 @@ int band_solve(Matrix &K, const void *fac, int prec, double *x_dev)
 int band_symv(const Matrix &K, const double *x_dev, double *y_dev)
 {
     Problem &pb = *K.pb;
+    ProfScope ps("band_symv", pb.stream);
     if (K.prec == KSH_FP64)
         hipLaunchKernelGGL(k_band_symv<double>, dim3(nblk(K.n, 256)), dim3(256), 0, pb.stream, (const double *)K.ab, K.n, K.bw,
                            x_dev, y_dev);

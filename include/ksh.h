@@ -133,6 +133,15 @@ int ksh_inertia(ksh_matrix *K, int64_t *n_negative, double *d_min);
 int ksh_vector_d(ksh_matrix *K, double *d_out);
 int ksh_set_inertia_guard(ksh_matrix *K, double guard);
 
+/* ---- measurement --------------------------------------------------------------
+ * No counterpart in the reference (it only accumulates wall time inside its lambdas,
+ * benchmarks/benchmark_Roof.cpp:316-343).  When enabled, every launch of the named
+ * kernels ("tangent_elem", "band_gather", "residual_elem", "vector_gather",
+ * "band_ldlt", "band_solve", "band_symv") is bracketed by HIP events on its stream. */
+int ksh_profile_enable(int on);
+int ksh_profile_reset(void);
+int ksh_profile_query(const char *kernel, int64_t *launches, double *total_ms);
+
 /* ---- solver drivers ---------------------------------------------------------
  * The reference's solver classes (gsALMCrisfield / gsALMRiks, gsStaticNewton +
  * gsControlDisplacement) run on the std::function operator surface bound to this
