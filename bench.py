@@ -101,7 +101,7 @@ def main():
     sp = ShellProblem(pt)
     kw = dict(method=0, max_iter=20, tol=1e-6, tol_f=1e-8, tol_u=1e-6, scaling=0.0, bifurcation_method=0,
               max_halvings=10)
-    mixed_kw = dict(max_refine=10, refine_tol=1e-12)
+    mixed_kw = dict(max_refine=6, refine_tol=1e-11)
     log = (lambda *a: print(*a, file=sys.stderr, flush=True)) if True else None
 
     # warm-up (allocations, code objects) for both precisions
@@ -134,6 +134,11 @@ def main():
 
     ok_steps = plain["steps_done"] == args.steps and mixed["steps_done"] == args.steps
     same_neg = bool(ok_steps and np.array_equal(plain["negatives"], mixed["negatives"]))
+    if ok_steps and not same_neg:
+        bad = np.flatnonzero(plain["negatives"] != mixed["negatives"])
+        log(f"[bench] negative counts differ at steps {bad.tolist()}: plain {plain['negatives'][bad].tolist()} "
+            f"mixed {mixed['negatives'][bad].tolist()}; lambda plain {plain['lambda'][bad].tolist()} "
+            f"mixed {mixed['lambda'][bad].tolist()}")
     value = mixed["steps_done"] / t_mixed
     plain_rate = plain["steps_done"] / t_plain
 

@@ -10,7 +10,7 @@ LIB_PATH = os.path.join(_HERE, "libksh.so")
 SUCCESS, NOT_CONVERGED, ASSEMBLY_ERROR, SOLVER_ERROR, NOT_STARTED, OTHER_ERROR = range(6)
 STATUS_NAMES = ["Success", "NotConverged", "AssemblyError", "SolverError", "NotStarted", "OtherError"]
 RES_STATIC, RES_ALM = 0, 1
-FP64, FP32_IR = 0, 1
+FP64, FP32_IR, FP32 = 0, 1, 2
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)

@@ -134,6 +134,8 @@ template <typename T> T *dalloc(size_t count, int &rc)
 }
 
 size_t elem_bytes(int prec) { return prec == KSH_FP64 ? sizeof(double) : sizeof(float); }
+int storage_prec(int mode) { return mode == KSH_FP32 ? KSH_FP32 : KSH_FP64; }
+int factor_prec(int mode) { return mode == KSH_FP64 ? KSH_FP64 : KSH_FP32; }
 
 int upload_free(Problem &pb, const double *host, double *dev)
 {
@@ -171,14 +173,20 @@ int inertia_fp64(Matrix &K, int64_t *neg, double *dmin)
     }
     Matrix tmp = K;
     tmp.prec = KSH_FP64;
+    tmp.sprec = KSH_FP64;
+    tmp.fprec = KSH_FP64;
     tmp.ab = K.fac64;
-    if (K.has_u) {
+    if (K.sprec == KSH_FP64) {
+        KSH_HIP(hipMemcpyAsync(K.fac64, K.ab, count * sizeof(double), hipMemcpyDeviceToDevice, pb.stream));
+    } else if (K.has_u) {
         if ((rc = launch_construct_solution(pb, K.u_at))) return rc;
         if ((rc = launch_tangent(pb, tmp))) return rc;
     } else {
-        if ((rc = band_convert(K.ab, K.prec, K.fac64, KSH_FP64, count, pb.stream))) return rc;
+        if ((rc = band_convert(K.ab, K.sprec, K.fac64, KSH_FP64, count, pb.stream))) return rc;
     }
     if ((rc = band_factor(tmp, K.fac64, KSH_FP64))) return rc;
+    K.use64 = true;
+    if (!neg && !dmin) return KSH_SUCCESS;
     return band_inertia(tmp, K.fac64, KSH_FP64, neg, dmin, nullptr, nullptr);
 }
 
@@ -404,26 +412,28 @@ int ksh_assemble_residual(ksh_problem *pb, const double *u_free, double lambda, 
 int ksh_matrix_create(ksh_problem *pb, int prec, ksh_matrix **out)
 {
     if (!pb || !out) return fail(KSH_OTHER_ERROR, "null argument");
-    if (prec != KSH_FP64 && prec != KSH_FP32_IR) return fail(KSH_OTHER_ERROR, "unknown precision");
+    if (prec != KSH_FP64 && prec != KSH_FP32_IR && prec != KSH_FP32) return fail(KSH_OTHER_ERROR, "unknown precision");
     ksh_matrix *K = new ksh_matrix();
     K->pb = pb;
     K->n = pb->nfree;
     K->bw = pb->bw;
     K->ld = (size_t)pb->bw + 1;
     K->prec = prec;
+    K->sprec = storage_prec(prec);
+    K->fprec = factor_prec(prec);
     int rc = 0;
-    size_t count = (size_t)K->n * K->ld, eb = elem_bytes(prec);
+    size_t count = (size_t)K->n * K->ld, eb = elem_bytes(K->sprec), fb = elem_bytes(K->fprec);
     auto bail = [&](int code) {
         destroy_matrix(K);
         delete K;
         return code;
     };
     if (!(K->ab = dalloc<unsigned char>(count * eb, rc))) return bail(rc);
-    if (!(K->fac = dalloc<unsigned char>(count * eb, rc))) return bail(rc);
+    if (!(K->fac = dalloc<unsigned char>(count * fb, rc))) return bail(rc);
     if (!(K->work = dalloc<double>((size_t)4 * K->n, rc))) return bail(rc);
     if (!(K->flag = dalloc<int>(4, rc))) return bail(rc);
     if (!(K->red = dalloc<double>(8, rc))) return bail(rc);
-    if (prec == KSH_FP32_IR && !(K->u_at = dalloc<double>(K->n, rc))) return bail(rc);
+    if (K->sprec == KSH_FP32 && !(K->u_at = dalloc<double>(K->n, rc))) return bail(rc);
     if (hipMemsetAsync(K->ab, 0, count * eb, pb->stream) != hipSuccess) return bail(fail(KSH_OTHER_ERROR, "memset"));
     *out = K;
     return KSH_SUCCESS;
@@ -454,7 +464,7 @@ int ksh_assemble_tangent_dev(ksh_problem *pb, const double *u_free_dev, ksh_matr
     int rc;
     if ((rc = launch_construct_solution(*pb, u_free_dev))) return fail(KSH_ASSEMBLY_ERROR, ksh_last_error());
     if ((rc = launch_tangent(*pb, *K))) return fail(KSH_ASSEMBLY_ERROR, ksh_last_error());
-    if (K->prec == KSH_FP32_IR) {
+    if (K->sprec == KSH_FP32) {
         KSH_HIP(hipMemcpyAsync(K->u_at, u_free_dev, sizeof(double) * K->n, hipMemcpyDeviceToDevice, pb->stream));
         K->has_u = true;
     }
@@ -485,7 +495,7 @@ int ksh_matrix_download(const ksh_matrix *K, double *ab_out)
     if (!K || !ab_out) return fail(KSH_OTHER_ERROR, "null argument");
     size_t count = (size_t)K->n * K->ld;
     hipStream_t s = K->pb->stream;
-    if (K->prec == KSH_FP64) {
+    if (K->sprec == KSH_FP64) {
         KSH_HIP(hipMemcpyAsync(ab_out, K->ab, count * sizeof(double), hipMemcpyDeviceToHost, s));
         KSH_HIP(hipStreamSynchronize(s));
     } else {
@@ -502,7 +512,7 @@ int ksh_matrix_upload(ksh_matrix *K, const double *ab_in)
     if (!K || !ab_in) return fail(KSH_OTHER_ERROR, "null argument");
     size_t count = (size_t)K->n * K->ld;
     hipStream_t s = K->pb->stream;
-    if (K->prec == KSH_FP64) {
+    if (K->sprec == KSH_FP64) {
         KSH_HIP(hipMemcpyAsync(K->ab, ab_in, count * sizeof(double), hipMemcpyHostToDevice, s));
         KSH_HIP(hipStreamSynchronize(s));
     } else {
@@ -532,13 +542,13 @@ int ksh_matrix_copy(ksh_matrix *dst, const ksh_matrix *src)
     if (dst->n != src->n || dst->bw != src->bw) return fail(KSH_OTHER_ERROR, "matrix shapes differ");
     size_t count = (size_t)dst->n * dst->ld;
     hipStream_t s = dst->pb->stream;
-    if (dst->prec == src->prec) {
-        KSH_HIP(hipMemcpyAsync(dst->ab, src->ab, count * elem_bytes(dst->prec), hipMemcpyDeviceToDevice, s));
+    if (dst->sprec == src->sprec) {
+        KSH_HIP(hipMemcpyAsync(dst->ab, src->ab, count * elem_bytes(dst->sprec), hipMemcpyDeviceToDevice, s));
     } else {
         int rc;
-        if ((rc = band_convert(src->ab, src->prec, dst->ab, dst->prec, count, s))) return rc;
+        if ((rc = band_convert(src->ab, src->sprec, dst->ab, dst->sprec, count, s))) return rc;
     }
-    dst->has_u = src->has_u && dst->prec == KSH_FP32_IR && src->prec == KSH_FP32_IR;
+    dst->has_u = src->has_u && dst->sprec == KSH_FP32 && src->sprec == KSH_FP32;
     if (dst->has_u) KSH_HIP(hipMemcpyAsync(dst->u_at, src->u_at, sizeof(double) * dst->n, hipMemcpyDeviceToDevice, s));
     ++dst->version;
     return KSH_SUCCESS;
@@ -567,9 +577,25 @@ int ksh_factor(ksh_matrix *K)
 {
     if (!K) return fail(KSH_OTHER_ERROR, "null argument");
     if (K->factored && K->factored_version == K->version) return K->info ? KSH_SOLVER_ERROR : KSH_SUCCESS;
-    int rc = band_factor(*K, K->fac, K->prec);
+    K->use64 = false;
+    int rc = band_factor(*K, K->fac, K->fprec);
     K->factored = true;
     K->factored_version = K->version;
+    if (K->fprec == KSH_FP32) {
+        // pivots too small relative to the largest one cannot be trusted in FP32: use an FP64
+        // factor of the same matrix for this version's solves and inertia
+        int64_t neg = 0;
+        double dmin = 0, dmaxabs = 0, dminabs = 0;
+        bool fallback = rc != KSH_SUCCESS;
+        if (!fallback) {
+            if ((rc = band_inertia(*K, K->fac, KSH_FP32, &neg, &dmin, &dmaxabs, &dminabs))) return rc;
+            fallback = !(dminabs >= K->guard * dmaxabs);
+        }
+        if (fallback) {
+            rc = inertia_fp64(*K, nullptr, nullptr);
+            K->info = (rc == KSH_SUCCESS) ? 0 : 1;
+        }
+    }
     return rc;
 }
 
@@ -580,8 +606,10 @@ int ksh_solve_dev(ksh_matrix *K, const double *b_dev, double *x_dev, int max_ref
     if (K->info) return fail(KSH_SOLVER_ERROR, "factorization failed");
     hipStream_t s = K->pb->stream;
     int n = K->n, rc;
+    const void *F = K->use64 ? (const void *)K->fac64 : K->fac;
+    const int fp = K->use64 ? KSH_FP64 : K->fprec;
     if (x_dev != b_dev) KSH_HIP(hipMemcpyAsync(x_dev, b_dev, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-    if ((rc = band_solve(*K, K->fac, K->prec, x_dev))) return rc;
+    if ((rc = band_solve(*K, F, fp, x_dev))) return rc;
     if (max_refine > 0) {
         double *bcopy = K->work + 2 * n, *r = K->work + 3 * n;
         if (x_dev == b_dev) return fail(KSH_OTHER_ERROR, "refinement needs distinct b and x");
@@ -596,7 +624,13 @@ int ksh_solve_dev(ksh_matrix *K, const double *b_dev, double *x_dev, int max_ref
             if ((rc = vec_norm2(s, n, r, K->red, &rn))) return rc;
             if (!(rn <= tol * bnorm) && !std::isfinite(rn)) return fail(KSH_SOLVER_ERROR, "refinement diverged");
             if (rn <= tol * bnorm) break;
-            if ((rc = band_solve(*K, K->fac, K->prec, r))) return rc;
+            if (it == max_refine - 1 && fp == KSH_FP32 && rn > 1e-6 * bnorm) {
+                // refinement stalled on the FP32 factor: solve with an FP64 factor instead
+                if ((rc = inertia_fp64(*K, nullptr, nullptr))) return rc;
+                KSH_HIP(hipMemcpyAsync(x_dev, bcopy, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+                return band_solve(*K, K->fac64, KSH_FP64, x_dev);
+            }
+            if ((rc = band_solve(*K, F, fp, r))) return rc;
             if ((rc = vec_axpy(s, n, 1.0, r, x_dev))) return rc;
         }
     }
@@ -632,9 +666,12 @@ int ksh_inertia(ksh_matrix *K, int64_t *n_negative, double *d_min)
     if ((rc = ksh_factor(K))) return rc;
     int64_t neg = 0;
     double dmin = 0, dmaxabs = 0, dminabs = 0;
-    if ((rc = band_inertia(*K, K->fac, K->prec, &neg, &dmin, &dmaxabs, &dminabs))) return rc;
-    if (K->prec == KSH_FP32_IR && dminabs < K->guard * dmaxabs) {
-        if ((rc = inertia_fp64(*K, &neg, &dmin))) return rc;
+    if (K->use64) {
+        Matrix tmp = *K;
+        tmp.sprec = tmp.fprec = KSH_FP64;
+        if ((rc = band_inertia(tmp, K->fac64, KSH_FP64, &neg, &dmin, &dmaxabs, &dminabs))) return rc;
+    } else {
+        if ((rc = band_inertia(*K, K->fac, K->fprec, &neg, &dmin, &dmaxabs, &dminabs))) return rc;
     }
     if (n_negative) *n_negative = neg;
     if (d_min) *d_min = dmin;
@@ -647,12 +684,14 @@ int ksh_vector_d(ksh_matrix *K, double *d_out)
     int rc;
     if ((rc = ksh_factor(K))) return rc;
     hipStream_t s = K->pb->stream;
-    size_t eb = elem_bytes(K->prec);
+    const int fp = K->use64 ? KSH_FP64 : K->fprec;
+    const void *F = K->use64 ? (const void *)K->fac64 : K->fac;
+    size_t eb = elem_bytes(fp);
     std::vector<unsigned char> tmp((size_t)K->n * eb);
-    KSH_HIP(hipMemcpy2DAsync(tmp.data(), eb, K->fac, K->ld * eb, eb, K->n, hipMemcpyDeviceToHost, s));
+    KSH_HIP(hipMemcpy2DAsync(tmp.data(), eb, F, K->ld * eb, eb, K->n, hipMemcpyDeviceToHost, s));
     KSH_HIP(hipStreamSynchronize(s));
     for (int i = 0; i < K->n; ++i)
-        d_out[i] = K->prec == KSH_FP64 ? ((double *)tmp.data())[i] : (double)((float *)tmp.data())[i];
+        d_out[i] = fp == KSH_FP64 ? ((double *)tmp.data())[i] : (double)((float *)tmp.data())[i];
     return KSH_SUCCESS;
 }
 

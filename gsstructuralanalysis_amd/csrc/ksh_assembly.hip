@@ -612,7 +612,7 @@ int launch_tangent(Problem &pb, Matrix &K)
 {
     ElemArgs ar = elem_args(pb);
     size_t slots = (size_t)K.n * K.ld;
-    if (K.prec == KSH_FP64) {
+    if (K.sprec == KSH_FP64) {
         double *Ke = (double *)pb.Ke;
         {
             ProfScope ps("tangent_elem", pb.stream);
@@ -647,7 +647,7 @@ int launch_mass(Problem &pb, Matrix &M)
 {
     ElemArgs ar = elem_args(pb);
     size_t slots = (size_t)M.n * M.ld;
-    if (M.prec == KSH_FP64) {
+    if (M.sprec == KSH_FP64) {
         double *Ke = (double *)pb.Ke;
         if (pb.p == 2)
             hipLaunchKernelGGL((k_mass_elem<2, double>), dim3(pb.nel), dim3(256), 0, pb.stream, ar, pb.rho, Ke);

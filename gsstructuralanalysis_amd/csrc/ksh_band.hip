@@ -156,73 +156,132 @@ __global__ __launch_bounds__(kThreads) void k_band_ldlt(T *fac, int n, int bw, i
     if (tid == 0) *info = s_info;
 }
 
-// Forward / diagonal / backward substitution with the factored band (one workgroup).
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_band_solve(const T *fac, int n, int bw, double *x)
+// Forward / diagonal / backward substitution with the factored band (one workgroup per matrix).
+// Blocks of SB = 64 rows.  Forward block: every thread accumulates part of the left-looking
+// product sum_{j<r0} L[i][j] y[j] for the block's rows with coalesced loads (lanes along i within
+// a band column, 16 loads issued before use), the 64x64 diagonal block is staged through LDS into
+// registers (lane = row), then wave 0 finishes the block with a 64-step recurrence whose
+// broadcasts are v_readlane (no LDS on the dependency chain).  Backward block: each wave reduces
+// 16 columns' dot products sum_{i>=c0+64} L[i][j] x[i] with lanes along i, then wave 0 runs the
+// transposed recurrence.  The band is read exactly twice.
+constexpr int SB = 64;
+constexpr int SU = 16;  // loads in flight per thread in the forward product
+
+__device__ inline double readlane_f64(double v, int lane)
 {
-    __shared__ double ys[NB];
-    __shared__ double part[NB][NB + 1];
+    int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+template <typename T>
+__device__ void band_solve_body(const T *__restrict__ fac, int n, int bw, double *__restrict__ x)
+{
+    __shared__ double sdiag[SB][SB + 1];  // sdiag[jj][ii] = L[r0+ii][r0+jj]
+    __shared__ double spart[4][SB];
     const size_t ld = (size_t)bw + 1;
     const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
-    // forward: L y = b
-    for (int c0 = 0; c0 < n; c0 += NB) {
-        const int w = min(NB, n - c0);
-        if (wave == 0) {
-            double y = (lane < w) ? x[c0 + lane] : 0.0;
-            for (int jj = 0; jj < w; ++jj) {
-                double yj = __shfl(y, jj);
-                if (lane > jj && lane < w && lane - jj <= bw) y -= (double)fac[(size_t)(c0 + jj) * ld + (lane - jj)] * yj;
+    // ---- forward: L y = b ----
+    for (int r0 = 0; r0 < n; r0 += SB) {
+        const int w = min(SB, n - r0);
+        for (int idx = tid; idx < SB * SB; idx += kThreads) {
+            int jj = idx / SB, ii = idx % SB;
+            double v = 0.0;
+            if (jj < w && ii < w && ii > jj && ii - jj <= bw) v = (double)fac[(size_t)(r0 + jj) * ld + (ii - jj)];
+            sdiag[jj][ii] = v;
+        }
+        {
+            const int i = r0 + lane;
+            const int jlo = max(0, i - bw);
+            double acc = 0.0;
+            const int jbeg = max(0, r0 - bw) + wave;
+            for (int base = jbeg; base < r0; base += 4 * SU) {
+                double lv[SU], xv[SU];
+#pragma unroll
+                for (int u = 0; u < SU; ++u) {
+                    int j = base + 4 * u;
+                    bool ok = lane < w && j < r0 && j >= jlo;
+                    lv[u] = ok ? (double)fac[(size_t)j * ld + (i - j)] : 0.0;
+                    xv[u] = (j < r0) ? x[j] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < SU; ++u) acc += lv[u] * xv[u];
             }
-            if (lane < w) {
-                x[c0 + lane] = y;
-                ys[lane] = y;
-            }
+            spart[wave][lane] = acc;
         }
         __syncthreads();
-        const int rend = min(n - 1, c0 + w - 1 + bw);
-        for (int i = c0 + w + tid; i <= rend; i += kThreads) {
-            double acc = 0.0;
-            for (int jj = 0; jj < w; ++jj) {
-                int off = i - c0 - jj;
-                if (off <= bw) acc += (double)fac[(size_t)(c0 + jj) * ld + off] * ys[jj];
+        if (wave == 0) {
+            double row[SB];
+#pragma unroll
+            for (int jj = 0; jj < SB; ++jj) row[jj] = sdiag[jj][lane];
+            double y = 0.0;
+            if (lane < w) y = x[r0 + lane] - (spart[0][lane] + spart[1][lane] + spart[2][lane] + spart[3][lane]);
+#pragma unroll
+            for (int jj = 0; jj < SB; ++jj) {
+                double yj = readlane_f64(y, jj);
+                if (lane > jj) y -= row[jj] * yj;
             }
-            x[i] -= acc;
+            if (lane < w) x[r0 + lane] = y;
         }
         __syncthreads();
     }
-    // diagonal
+    // ---- diagonal ----
     for (int i = tid; i < n; i += kThreads) x[i] /= (double)fac[(size_t)i * ld];
     __syncthreads();
-    // backward: L^T x = y
-    const int nblk = (n + NB - 1) / NB;
+    // ---- backward: L^T x = y ----
+    const int nblk = (n + SB - 1) / SB;
     for (int bk = nblk - 1; bk >= 0; --bk) {
-        const int c0 = bk * NB, w = min(NB, n - c0);
-        // contributions of rows below the panel: s_jj = sum_{i >= c0+w} L[i][c0+jj] x_i
-        {
-            const int jj = tid / 16, sub = tid % 16;
-            double acc = 0.0;
-            if (jj < w) {
-                const int ilo = c0 + w, ihi = min(n - 1, c0 + jj + bw);
-                for (int i = ilo + sub; i <= ihi; i += 16) acc += (double)fac[(size_t)(c0 + jj) * ld + (i - c0 - jj)] * x[i];
+        const int c0 = bk * SB, w = min(SB, n - c0);
+        for (int idx = tid; idx < SB * SB; idx += kThreads) {
+            int jj = idx / SB, ii = idx % SB;
+            double v = 0.0;
+            if (jj < w && ii < w && ii > jj && ii - jj <= bw) v = (double)fac[(size_t)(c0 + jj) * ld + (ii - jj)];
+            sdiag[jj][ii] = v;
+        }
+        double acc[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) acc[c] = 0.0;
+        const int ibase = c0 + w;
+        const int iend = min(n, c0 + w + bw);
+        for (int i = ibase + lane; i < iend; i += 64) {
+            double xi = x[i];
+            double lv[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                int jj = 16 * wave + c;
+                int j = c0 + jj;
+                lv[c] = (jj < w && i - j <= bw) ? (double)fac[(size_t)j * ld + (i - j)] : 0.0;
             }
-            part[jj][sub] = acc;
+#pragma unroll
+            for (int c = 0; c < 16; ++c) acc[c] += lv[c] * xi;
+        }
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            double v = acc[c];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+            if (lane == 0) spart[0][16 * wave + c] = v;
         }
         __syncthreads();
         if (wave == 0) {
-            double xv = 0.0;
-            if (lane < w) {
-                double s = 0.0;
-                for (int q = 0; q < 16; ++q) s += part[lane][q];
-                xv = x[c0 + lane] - s;
-            }
-            for (int jj = w - 1; jj >= 0; --jj) {
-                double xj = __shfl(xv, jj);
-                if (lane < jj && jj - lane <= bw) xv -= (double)fac[(size_t)(c0 + lane) * ld + (jj - lane)] * xj;
+            double col[SB];  // col[jj] = L[c0+jj][c0+lane]
+#pragma unroll
+            for (int jj = 0; jj < SB; ++jj) col[jj] = sdiag[lane][jj];
+            double xv = (lane < w) ? x[c0 + lane] - spart[0][lane] : 0.0;
+#pragma unroll
+            for (int jj = SB - 1; jj >= 0; --jj) {
+                double xj = readlane_f64(xv, jj);
+                if (lane < jj) xv -= col[jj] * xj;
             }
             if (lane < w) x[c0 + lane] = xv;
         }
         __syncthreads();
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_band_solve(const T *fac, int n, int bw, double *x)
+{
+    band_solve_body<T>(fac, n, bw, x);
 }
 
 // y = K x with K symmetric, lower band stored (FP64 accumulation).
@@ -330,10 +389,10 @@ int band_factor(Matrix &K, void *fac, int prec)
     Problem &pb = *K.pb;
     size_t bytes = (size_t)K.n * K.ld * (prec == KSH_FP64 ? sizeof(double) : sizeof(float));
     if (fac != K.ab) {
-        if (prec == K.prec) {
+        if (prec == K.sprec) {
             KSH_HIP(hipMemcpyAsync(fac, K.ab, bytes, hipMemcpyDeviceToDevice, pb.stream));
         } else {
-            int rc = band_convert(K.ab, K.prec, fac, prec, (size_t)K.n * K.ld, pb.stream);
+            int rc = band_convert(K.ab, K.sprec, fac, prec, (size_t)K.n * K.ld, pb.stream);
             if (rc) return rc;
         }
     }
@@ -374,7 +433,7 @@ int band_symv(const Matrix &K, const double *x_dev, double *y_dev)
 {
     Problem &pb = *K.pb;
     ProfScope ps("band_symv", pb.stream);
-    if (K.prec == KSH_FP64)
+    if (K.sprec == KSH_FP64)
         hipLaunchKernelGGL(k_band_symv<double>, dim3(nblk(K.n, 256)), dim3(256), 0, pb.stream, (const double *)K.ab, K.n, K.bw,
                            x_dev, y_dev);
     else
@@ -412,13 +471,13 @@ int band_axpby(Matrix &Y, double alpha, double beta, const Matrix &X)
 {
     Problem &pb = *Y.pb;
     size_t count = (size_t)Y.n * Y.ld;
-    if (Y.prec == KSH_FP64 && X.prec == KSH_FP64)
+    if (Y.sprec == KSH_FP64 && X.sprec == KSH_FP64)
         hipLaunchKernelGGL((k_band_axpby<double, double>), dim3(nblk(count, 256)), dim3(256), 0, pb.stream, (double *)Y.ab,
                            (const double *)X.ab, count, alpha, beta);
-    else if (Y.prec == KSH_FP64)
+    else if (Y.sprec == KSH_FP64)
         hipLaunchKernelGGL((k_band_axpby<double, float>), dim3(nblk(count, 256)), dim3(256), 0, pb.stream, (double *)Y.ab,
                            (const float *)X.ab, count, alpha, beta);
-    else if (X.prec == KSH_FP64)
+    else if (X.sprec == KSH_FP64)
         hipLaunchKernelGGL((k_band_axpby<float, double>), dim3(nblk(count, 256)), dim3(256), 0, pb.stream, (float *)Y.ab,
                            (const double *)X.ab, count, alpha, beta);
     else
@@ -430,10 +489,10 @@ int band_axpby(Matrix &Y, double alpha, double beta, const Matrix &X)
 
 int band_convert(const void *src, int src_prec, void *dst, int dst_prec, size_t count, hipStream_t s)
 {
-    if (src_prec == KSH_FP64 && dst_prec == KSH_FP32_IR)
+    if (src_prec == KSH_FP64 && dst_prec == KSH_FP32)
         hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(count, 256)), dim3(256), 0, s, (float *)dst, (const double *)src,
                            count);
-    else if (src_prec == KSH_FP32_IR && dst_prec == KSH_FP64)
+    else if (src_prec == KSH_FP32 && dst_prec == KSH_FP64)
         hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(count, 256)), dim3(256), 0, s, (double *)dst, (const float *)src,
                            count);
     else

@@ -68,11 +68,14 @@ struct Problem {
 struct Matrix {
     Problem *pb = nullptr;
     int n = 0, bw = 0, prec = KSH_FP64;
+    int sprec = KSH_FP64;          // storage precision of the assembled values (KSH_FP64 / KSH_FP32)
+    int fprec = KSH_FP64;          // precision of the LDL^T factor
     size_t ld = 0;                 // bw + 1
-    void *ab = nullptr;            // assembled values (FP64 or FP32)
-    void *fac = nullptr;           // L and D after factorization (same precision)
-    double *fac64 = nullptr;       // FP64 refactorization used by the inertia guard
-    double *u_at = nullptr;        // free vector the matrix was assembled at (FP32_IR guard)
+    void *ab = nullptr;            // assembled values (sprec)
+    void *fac = nullptr;           // L and D after factorization (fprec)
+    double *fac64 = nullptr;       // FP64 refactorization (near-singular FP32 pivots)
+    bool use64 = false;            // solves of the current factor use fac64
+    double *u_at = nullptr;        // free vector the matrix was assembled at (FP32 storage)
     bool has_u = false;
     bool factored = false;
     uint64_t version = 0;          // bumped on every change of ab

@@ -42,8 +42,12 @@ enum ksh_residual_conv {
 
 /* matrix / factor precision */
 enum ksh_precision {
-    KSH_FP64 = 0,     /* FP64 assembly, FP64 band LDL^T                         */
-    KSH_FP32_IR = 1   /* FP32 assembly and band LDL^T on MFMA, FP64 refinement   */
+    KSH_FP64 = 0,     /* FP64 assembly, FP64 band LDL^T                                   */
+    KSH_FP32_IR = 1,  /* FP64 assembly, FP32 band LDL^T on MFMA, refinement residuals in
+                         FP64 against the FP64 K; near-singular FP32 pivots (|d|min <
+                         guard |d|max) switch that factor to FP64                         */
+    KSH_FP32 = 2      /* FP32 element GEMMs and FP32 band (K itself rounded to FP32),
+                         FP32 LDL^T, refinement residuals in FP64 arithmetic on K32       */
 };
 
 typedef struct ksh_problem ksh_problem;

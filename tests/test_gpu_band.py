@@ -60,13 +60,18 @@ def test_inertia_matches_eigenvalues(k, ksh):
     assert (dmin < 0) == (ref > 0)
 
 
-def test_fp32_factor_with_refinement(ksh):
+@pytest.mark.parametrize("mode", [1, 2])
+def test_fp32_factor_with_refinement(mode, ksh):
+    """mode 1: FP64 K, FP32 LDL^T, FP64 refinement residuals -> the FP64 solution of K64;
+    mode 2: K rounded to FP32 -> the FP64-accurate solution of K32."""
     pt, sp, o = _problem(3, 6)
     u = np.random.default_rng(4).standard_normal(o.nfree) * 1e-3
-    K32 = sp.tangent(u, precision=1)
+    K32 = sp.tangent(u, precision=mode)
     K32.factor()
     b = np.random.default_rng(5).standard_normal(o.nfree)
-    A32 = band_to_dense(K32.download(), o.nfree, o.bw)  # the FP32-assembled values
+    A32 = band_to_dense(K32.download(), o.nfree, o.bw)  # the stored K values
+    if mode == 1:
+        assert np.abs(K32.download() - o.tangent_band(u)).max() <= 1e-11 * np.abs(o.tangent_band(u)).max()
     xr = sl.solve(A32, b, assume_a="sym")
     x0 = K32.solve(b, max_refine=0)
     x = K32.solve(b, max_refine=20, tol=1e-13)
