@@ -139,6 +139,31 @@ private:
 };
 
 // ------------------------------------------------------------------------------------------
+// Executor of the device operations behind the operator surface.  The default runs each call
+// on libksh.so directly; a batch engine supplies one that defers the call into a batched launch
+// over many problems (the calling solver blocks until its result is ready).
+struct gsExec {
+    virtual ~gsExec() = default;
+    virtual int tangent(ksh_problem *pb, const double *u, ksh_matrix *K) { return ksh_assemble_tangent(pb, u, K); }
+    virtual int residual(ksh_problem *pb, const double *u, double lam, int conv, double *R)
+    {
+        return ksh_assemble_residual(pb, u, lam, conv, R);
+    }
+    virtual int factor(ksh_matrix *K) { return ksh_factor(K); }
+    virtual int solve(ksh_matrix *K, const double *b, double *x, int max_refine, double tol)
+    {
+        return ksh_solve(K, b, x, max_refine, tol);
+    }
+    virtual int inertia(ksh_matrix *K, int64_t *neg, double *dmin) { return ksh_inertia(K, neg, dmin); }
+    virtual int matvec(const ksh_matrix *K, const double *x, double *y) { return ksh_matvec(K, x, y); }
+    virtual int vector_d(ksh_matrix *K, double *d) { return ksh_vector_d(K, d); }
+    static gsExec *direct()
+    {
+        static gsExec e;
+        return &e;
+    }
+};
+
 // Device band matrix handle + pool
 class gsMatrixPool;
 
@@ -146,6 +171,7 @@ struct gsMatrixStore {
     ksh_matrix *K = nullptr;
     ksh_problem *pb = nullptr;
     int prec = KSH_FP64;
+    gsExec *exec = gsExec::direct();
     ~gsMatrixStore()
     {
         if (K) ksh_matrix_destroy(K);
@@ -161,6 +187,7 @@ public:
     bool valid() const { return m_s && m_s->K; }
     ksh_matrix *handle() const { return m_s ? m_s->K : nullptr; }
     ksh_problem *problem() const { return m_s ? m_s->pb : nullptr; }
+    gsExec *exec() const { return m_s ? m_s->exec : gsExec::direct(); }
     std::shared_ptr<gsMatrixStore> store() const { return m_s; }
     index_t rows() const
     {
@@ -174,7 +201,7 @@ public:
     gsVector<T> operator*(const gsVector<T> &x) const
     {
         gsVector<T> y(rows());
-        if (ksh_matvec(handle(), x.data(), y.data()) != KSH_SUCCESS) throw std::runtime_error(ksh_last_error());
+        if (exec()->matvec(handle(), x.data(), y.data()) != KSH_SUCCESS) throw std::runtime_error(ksh_last_error());
         return y;
     }
 
@@ -186,7 +213,7 @@ private:
 // gsSparseMatrix refers to it any more.
 class gsMatrixPool {
 public:
-    gsMatrixPool(ksh_problem *pb, int prec) : m_pb(pb), m_prec(prec) {}
+    gsMatrixPool(ksh_problem *pb, int prec, gsExec *exec = gsExec::direct()) : m_pb(pb), m_prec(prec), m_exec(exec) {}
     template <class T> gsSparseMatrix<T> acquire()
     {
         for (auto &s : m_stores)
@@ -194,6 +221,7 @@ public:
         auto s = std::make_shared<gsMatrixStore>();
         s->pb = m_pb;
         s->prec = m_prec;
+        s->exec = m_exec;
         if (ksh_matrix_create(m_pb, m_prec, &s->K) != KSH_SUCCESS) throw std::runtime_error(ksh_last_error());
         m_stores.push_back(s);
         return gsSparseMatrix<T>(s);
@@ -201,10 +229,12 @@ public:
     size_t size() const { return m_stores.size(); }
     ksh_problem *problem() const { return m_pb; }
     int precision() const { return m_prec; }
+    gsExec *exec() const { return m_exec; }
 
 private:
     ksh_problem *m_pb;
     int m_prec;
+    gsExec *m_exec;
     std::vector<std::shared_ptr<gsMatrixStore>> m_stores;
 };
 
@@ -232,13 +262,13 @@ public:
     virtual void compute(const gsSparseMatrix<T> &K)
     {
         m_K = K;
-        int rc = ksh_factor(K.handle());
+        int rc = K.exec()->factor(K.handle());
         m_info = rc == KSH_SUCCESS ? gsEigen::Success : gsEigen::NumericalIssue;
     }
     virtual gsVector<T> solve(const gsVector<T> &b) const
     {
         gsVector<T> x(b.rows());
-        int rc = ksh_solve(m_K.handle(), b.data(), x.data(), settings().max_refine, settings().refine_tol);
+        int rc = m_K.exec()->solve(m_K.handle(), b.data(), x.data(), settings().max_refine, settings().refine_tol);
         if (rc != KSH_SUCCESS) throw std::runtime_error(ksh_last_error());
         return x;
     }
@@ -263,7 +293,7 @@ public:
     gsVector<T> vectorD() const
     {
         gsVector<T> d(this->m_K.rows());
-        if (ksh_vector_d(this->m_K.handle(), d.data()) != KSH_SUCCESS) throw std::runtime_error(ksh_last_error());
+        if (this->m_K.exec()->vector_d(this->m_K.handle(), d.data()) != KSH_SUCCESS) throw std::runtime_error(ksh_last_error());
         return d;
     }
     // inertia with the FP32 near-zero-pivot guard (FP64 refactorization)
@@ -271,7 +301,7 @@ public:
     {
         int64_t neg = 0;
         double dm = 0;
-        if (ksh_inertia(this->m_K.handle(), &neg, &dm) != KSH_SUCCESS) throw std::runtime_error(ksh_last_error());
+        if (this->m_K.exec()->inertia(this->m_K.handle(), &neg, &dm) != KSH_SUCCESS) throw std::runtime_error(ksh_last_error());
         if (dmin) *dmin = (T)dm;
         return neg;
     }

@@ -3,7 +3,8 @@
 // These lambdas are what the reference drivers write around gsThinShellAssembler
 // (benchmarks/benchmark_Roof.cpp:323-344, tutorials/nonlinear_shell_static.cpp:120-136,
 // examples/example_DisplacementControl.cpp:236-254): constructSolution + assemble* + the sign
-// convention of the residual.  Here they call the device path; K never leaves the GPU.
+// convention of the residual.  Here they call the device path through an executor (direct, or a
+// batch engine that merges the calls of many problems into one launch); K never leaves the GPU.
 #pragma once
 
 #include "gsStructuralAnalysisTypes.h"
@@ -12,7 +13,8 @@ namespace gismo {
 
 class gsShellOps {
 public:
-    gsShellOps(ksh_problem *pb, int precision) : m_pb(pb), m_pool(pb, precision)
+    gsShellOps(ksh_problem *pb, int precision, gsExec *exec = gsExec::direct())
+        : m_pb(pb), m_exec(exec), m_pool(pb, precision, exec)
     {
         int64_t n = 0;
         ksh_num_free_dofs(pb, &n);
@@ -22,6 +24,7 @@ public:
     index_t numDofs() const { return m_n; }
     ksh_problem *problem() const { return m_pb; }
     gsMatrixPool &pool() { return m_pool; }
+    gsExec *exec() const { return m_exec; }
 
     /// F = rhs() of assemble() (load at lambda = 1)
     gsVector<real_t> force() const
@@ -36,7 +39,7 @@ public:
     {
         return [this](gsVector<real_t> const &u, gsSparseMatrix<real_t> &m) -> bool {
             m = m_pool.acquire<real_t>();
-            return ksh_assemble_tangent(m_pb, u.data(), m.handle()) == KSH_SUCCESS;
+            return m_exec->tangent(m_pb, u.data(), m.handle()) == KSH_SUCCESS;
         };
     }
 
@@ -45,7 +48,7 @@ public:
     {
         return [this](gsVector<real_t> const &u, const real_t lam, gsVector<real_t> &r) -> bool {
             r.resize(m_n);
-            return ksh_assemble_residual(m_pb, u.data(), lam, KSH_RES_ALM, r.data()) == KSH_SUCCESS;
+            return m_exec->residual(m_pb, u.data(), lam, KSH_RES_ALM, r.data()) == KSH_SUCCESS;
         };
     }
 
@@ -55,7 +58,7 @@ public:
     {
         return [this](gsVector<real_t> const &u, const real_t lam, gsVector<real_t> &r) -> bool {
             r.resize(m_n);
-            return ksh_assemble_residual(m_pb, u.data(), lam, KSH_RES_STATIC, r.data()) == KSH_SUCCESS;
+            return m_exec->residual(m_pb, u.data(), lam, KSH_RES_STATIC, r.data()) == KSH_SUCCESS;
         };
     }
 
@@ -64,7 +67,7 @@ public:
     {
         return [this](gsVector<real_t> const &u, gsVector<real_t> &r) -> bool {
             r.resize(m_n);
-            return ksh_assemble_residual(m_pb, u.data(), 1.0, KSH_RES_STATIC, r.data()) == KSH_SUCCESS;
+            return m_exec->residual(m_pb, u.data(), 1.0, KSH_RES_STATIC, r.data()) == KSH_SUCCESS;
         };
     }
 
@@ -79,6 +82,7 @@ public:
 
 private:
     ksh_problem *m_pb;
+    gsExec *m_exec;
     gsMatrixPool m_pool;
     index_t m_n = 0;
 };

@@ -147,7 +147,8 @@ void destroy_problem(Problem *pb)
 {
     if (!pb) return;
     void *ptrs[] = {pb->ctrl0, pb->free_idx, pb->free2full, pb->tabx, pb->taby, pb->wq, pb->gpgeo,
-                    pb->u_full, pb->u_free, pb->r_free, pb->Ke, pb->Fe, pb->F, pb->pl_N, pb->pl_span, pb->pl_F};
+                    pb->u_full, pb->u_free, pb->r_free, pb->Ke, pb->Fe, pb->F, pb->pl_N, pb->pl_span, pb->pl_F,
+                    pb->d_desc};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (pb->stream) (void)hipStreamDestroy(pb->stream);
@@ -184,13 +185,20 @@ int inertia_fp64(Matrix &K, int64_t *neg, double *dmin)
     } else {
         if ((rc = band_convert(K.ab, K.sprec, K.fac64, KSH_FP64, count, pb.stream))) return rc;
     }
-    if ((rc = band_factor(tmp, K.fac64, KSH_FP64))) return rc;
+    double st[4];
+    if ((rc = band_factor(tmp, K.fac64, KSH_FP64, st))) return rc;
     K.use64 = true;
-    if (!neg && !dmin) return KSH_SUCCESS;
-    return band_inertia(tmp, K.fac64, KSH_FP64, neg, dmin, nullptr, nullptr);
+    for (int i = 0; i < 4; ++i) K.fstats[i] = st[i];
+    K.has_stats = true;
+    if (neg) *neg = (int64_t)st[0];
+    if (dmin) *dmin = st[1];
+    return KSH_SUCCESS;
 }
 
 }  // namespace
+
+int refactor_fp64(Matrix &K) { return inertia_fp64(K, nullptr, nullptr); }
+
 }  // namespace ksh
 
 using namespace ksh;
@@ -332,6 +340,7 @@ int ksh_problem_create(const ksh_patch *patch, const ksh_material *mat, const ks
     if (!(P.pl_N = dalloc<double>(plN.size(), rc))) return bail(rc);
     if (!(P.pl_span = dalloc<int>(plspan.size(), rc))) return bail(rc);
     if (!(P.pl_F = dalloc<double>(plF.size(), rc))) return bail(rc);
+    if (!(P.d_desc = dalloc<InstDesc>(1, rc))) return bail(rc);
     auto up = [&](void *dst, const void *src, size_t bytes) {
         return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, P.stream) == hipSuccess;
     };
@@ -578,19 +587,16 @@ int ksh_factor(ksh_matrix *K)
     if (!K) return fail(KSH_OTHER_ERROR, "null argument");
     if (K->factored && K->factored_version == K->version) return K->info ? KSH_SOLVER_ERROR : KSH_SUCCESS;
     K->use64 = false;
-    int rc = band_factor(*K, K->fac, K->fprec);
+    double st[4] = {0, 0, 0, 0};
+    int rc = band_factor(*K, K->fac, K->fprec, st);
     K->factored = true;
     K->factored_version = K->version;
+    for (int i = 0; i < 4; ++i) K->fstats[i] = st[i];
+    K->has_stats = rc == KSH_SUCCESS;
     if (K->fprec == KSH_FP32) {
         // pivots too small relative to the largest one cannot be trusted in FP32: use an FP64
         // factor of the same matrix for this version's solves and inertia
-        int64_t neg = 0;
-        double dmin = 0, dmaxabs = 0, dminabs = 0;
-        bool fallback = rc != KSH_SUCCESS;
-        if (!fallback) {
-            if ((rc = band_inertia(*K, K->fac, KSH_FP32, &neg, &dmin, &dmaxabs, &dminabs))) return rc;
-            fallback = !(dminabs >= K->guard * dmaxabs);
-        }
+        bool fallback = rc != KSH_SUCCESS || !(st[3] >= K->guard * st[2]);
         if (fallback) {
             rc = inertia_fp64(*K, nullptr, nullptr);
             K->info = (rc == KSH_SUCCESS) ? 0 : 1;
@@ -666,7 +672,10 @@ int ksh_inertia(ksh_matrix *K, int64_t *n_negative, double *d_min)
     if ((rc = ksh_factor(K))) return rc;
     int64_t neg = 0;
     double dmin = 0, dmaxabs = 0, dminabs = 0;
-    if (K->use64) {
+    if (K->has_stats) {  // statistics reduced by the factorization kernel itself
+        neg = (int64_t)K->fstats[0];
+        dmin = K->fstats[1];
+    } else if (K->use64) {
         Matrix tmp = *K;
         tmp.sprec = tmp.fprec = KSH_FP64;
         if ((rc = band_inertia(tmp, K->fac64, KSH_FP64, &neg, &dmin, &dmaxabs, &dminabs))) return rc;

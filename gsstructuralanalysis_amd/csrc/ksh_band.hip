@@ -36,10 +36,23 @@ constexpr int kThreads = 256;
 
 __host__ __device__ inline int panel_rows(int bw) { return ((bw + 15) / 16) * 16 + 32; }
 
+// One matrix of a batched LDL^T / solve launch (blockIdx.x = matrix).
+struct BandDesc {
+    void *fac;       // factor storage (in place)
+    int n, bw;
+    int *info;       // 0 or 1-based zero-pivot column
+    double *stats;   // [4]: negatives, min d, max |d|, min |d|
+    double *x;       // solve: right-hand side in, solution out
+    int active;      // 0: skip this matrix
+    int pad_;
+};
+
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_band_ldlt(T *fac, int n, int bw, int *info)
+__device__ void pivot_stats(const T *fac, int n, int bw, double *stats);
+
+template <typename T>
+__device__ void band_ldlt_body(T *fac, int n, int bw, int *info, double *stats, unsigned char *smem)
 {
-    extern __shared__ __align__(16) unsigned char smem[];
     const int PR = panel_rows(bw);
     T *pan = (T *)smem;                     // [16][PR]
     T *Wdd = pan + (size_t)NB * PR;         // [16][16] d_kk * L_dd[jj][kk]
@@ -154,6 +167,60 @@ __global__ __launch_bounds__(kThreads) void k_band_ldlt(T *fac, int n, int bw, i
         if (s_info != 0) break;
     }
     if (tid == 0) *info = s_info;
+    __syncthreads();
+    if (stats) pivot_stats<T>(fac, n, bw, stats);
+}
+
+// negatives, min d, max |d|, min |d| of the factored diagonal (whole workgroup)
+template <typename T>
+__device__ void pivot_stats(const T *fac, int n, int bw, double *stats)
+{
+    __shared__ double sneg[kThreads], smin[kThreads], smax[kThreads], sminabs[kThreads];
+    const size_t ld = (size_t)bw + 1;
+    double c = 0.0, mn = INFINITY, mxa = 0.0, mna = INFINITY;
+    for (int i = threadIdx.x; i < n; i += kThreads) {
+        double d = (double)fac[(size_t)i * ld];
+        if (d < 0.0) c += 1.0;
+        mn = fmin(mn, d);
+        mxa = fmax(mxa, fabs(d));
+        mna = fmin(mna, fabs(d));
+    }
+    sneg[threadIdx.x] = c;
+    smin[threadIdx.x] = mn;
+    smax[threadIdx.x] = mxa;
+    sminabs[threadIdx.x] = mna;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            sneg[threadIdx.x] += sneg[threadIdx.x + s];
+            smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + s]);
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+            sminabs[threadIdx.x] = fmin(sminabs[threadIdx.x], sminabs[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        stats[0] = sneg[0];
+        stats[1] = smin[0];
+        stats[2] = smax[0];
+        stats[3] = sminabs[0];
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_band_ldlt(T *fac, int n, int bw, int *info, double *stats)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    band_ldlt_body<T>(fac, n, bw, info, stats, smem);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_band_ldlt_batched(const BandDesc *descs)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const BandDesc &d = descs[blockIdx.x];
+    if (!d.active) return;
+    band_ldlt_body<T>((T *)d.fac, d.n, d.bw, d.info, d.stats, smem);
 }
 
 // Forward / diagonal / backward substitution with the factored band (one workgroup per matrix).
@@ -284,6 +351,14 @@ __global__ __launch_bounds__(kThreads) void k_band_solve(const T *fac, int n, in
     band_solve_body<T>(fac, n, bw, x);
 }
 
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_band_solve_batched(const BandDesc *descs)
+{
+    const BandDesc &d = descs[blockIdx.x];
+    if (!d.active) return;
+    band_solve_body<T>((const T *)d.fac, d.n, d.bw, d.x);
+}
+
 // y = K x with K symmetric, lower band stored (FP64 accumulation).
 template <typename T>
 __global__ void k_band_symv(const T *ab, int n, int bw, const double *x, double *y)
@@ -337,6 +412,57 @@ __global__ __launch_bounds__(kThreads) void k_band_inertia(const T *fac, int n, 
     }
 }
 
+// Refinement residual of one matrix per workgroup: r = b - K x (K symmetric, lower band of the
+// assembled values, FP64 arithmetic), norms[0] = ||r||^2, norms[1] = ||b||^2.
+template <typename T>
+__device__ void refine_residual_body(const T *ab, int n, int bw, const double *b, const double *x, double *r,
+                                     double *norms)
+{
+    __shared__ double sr[kThreads], sb[kThreads];
+    const size_t ld = (size_t)bw + 1;
+    double accr = 0.0, accb = 0.0;
+    for (int i = threadIdx.x; i < n; i += kThreads) {
+        double acc = 0.0;
+        int dmax = min(bw, n - 1 - i);
+        for (int d = 0; d <= dmax; ++d) acc += (double)ab[(size_t)i * ld + d] * x[i + d];
+        int dmax2 = min(bw, i);
+        for (int d = 1; d <= dmax2; ++d) acc += (double)ab[(size_t)(i - d) * ld + d] * x[i - d];
+        double ri = b[i] - acc;
+        r[i] = ri;
+        accr += ri * ri;
+        accb += b[i] * b[i];
+    }
+    sr[threadIdx.x] = accr;
+    sb[threadIdx.x] = accb;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            sr[threadIdx.x] += sr[threadIdx.x + s];
+            sb[threadIdx.x] += sb[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        norms[0] = sr[0];
+        norms[1] = sb[0];
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_refine_residual_batched(const RefDesc *descs)
+{
+    const RefDesc &d = descs[blockIdx.x];
+    if (!d.active) return;
+    if (d.sprec == KSH_FP64) refine_residual_body<double>((const double *)d.ab, d.n, d.bw, d.b, d.x, d.r, d.norms);
+    else refine_residual_body<float>((const float *)d.ab, d.n, d.bw, d.b, d.x, d.r, d.norms);
+}
+
+__global__ __launch_bounds__(kThreads) void k_refine_update_batched(const RefDesc *descs)
+{
+    const RefDesc &d = descs[blockIdx.x];
+    if (!d.active) return;
+    for (int i = threadIdx.x; i < d.n; i += kThreads) d.x[i] += d.r[i];
+}
+
 template <typename TY, typename TX>
 __global__ void k_band_axpby(TY *y, const TX *x, size_t count, double alpha, double beta)
 {
@@ -382,9 +508,28 @@ template <typename T> size_t factor_smem(int bw)
     return sizeof(T) * ((size_t)NB * panel_rows(bw) + NB * NB + 2 * NB);
 }
 
+// LDS per CU on gfx950 is 160 KB; the factor kernels also hold static LDS (pivot flag and the
+// pivot-statistics reduction, 4 x 256 doubles), which counts against the same budget.
+constexpr size_t kStaticFactorLds = 4 * kThreads * sizeof(double) + 64;
+constexpr size_t kMaxLds = 160 * 1024 - kStaticFactorLds;
+
+// raise the dynamic-LDS limit of the factor kernels to `need` bytes (once per size increase)
+int allow_lds(size_t need)
+{
+    static size_t granted = 0;
+    if (need <= granted) return KSH_SUCCESS;
+    const void *fns[] = {(const void *)k_band_ldlt<double>, (const void *)k_band_ldlt<float>,
+                         (const void *)k_band_ldlt_batched<double>, (const void *)k_band_ldlt_batched<float>};
+    for (const void *f : fns) KSH_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+    granted = need;
+    return KSH_SUCCESS;
+}
+
+static_assert(sizeof(BandDesc) == kBandDescBytes, "BandDesc layout");
+
 }  // namespace
 
-int band_factor(Matrix &K, void *fac, int prec)
+int band_factor(Matrix &K, void *fac, int prec, double *stats_out)
 {
     Problem &pb = *K.pb;
     size_t bytes = (size_t)K.n * K.ld * (prec == KSH_FP64 ? sizeof(double) : sizeof(float));
@@ -397,23 +542,91 @@ int band_factor(Matrix &K, void *fac, int prec)
         }
     }
     size_t smem = prec == KSH_FP64 ? factor_smem<double>(K.bw) : factor_smem<float>(K.bw);
-    if (smem > 160 * 1024)
+    if (smem > kMaxLds)
         return fail(KSH_SOLVER_ERROR, "band too wide for the one-workgroup factorization (LDS panel exceeds 160 KB)");
+    int rc;
+    if ((rc = allow_lds(smem))) return rc;
+    double *dstats = K.red + 4;
     if (prec == KSH_FP64) {
-        KSH_HIP(hipFuncSetAttribute((const void *)k_band_ldlt<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
         ProfScope ps("band_ldlt", pb.stream);
-        hipLaunchKernelGGL(k_band_ldlt<double>, dim3(1), dim3(kThreads), smem, pb.stream, (double *)fac, K.n, K.bw, K.flag);
+        hipLaunchKernelGGL(k_band_ldlt<double>, dim3(1), dim3(kThreads), smem, pb.stream, (double *)fac, K.n, K.bw, K.flag,
+                           dstats);
     } else {
-        KSH_HIP(hipFuncSetAttribute((const void *)k_band_ldlt<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
         ProfScope ps("band_ldlt", pb.stream);
-        hipLaunchKernelGGL(k_band_ldlt<float>, dim3(1), dim3(kThreads), smem, pb.stream, (float *)fac, K.n, K.bw, K.flag);
+        hipLaunchKernelGGL(k_band_ldlt<float>, dim3(1), dim3(kThreads), smem, pb.stream, (float *)fac, K.n, K.bw, K.flag,
+                           dstats);
     }
     KSH_HIP(hipGetLastError());
     int info = 0;
+    double st[4];
     KSH_HIP(hipMemcpyAsync(&info, K.flag, sizeof(int), hipMemcpyDeviceToHost, pb.stream));
+    KSH_HIP(hipMemcpyAsync(st, dstats, sizeof(st), hipMemcpyDeviceToHost, pb.stream));
     KSH_HIP(hipStreamSynchronize(pb.stream));
     K.info = info;
+    if (stats_out)
+        for (int i = 0; i < 4; ++i) stats_out[i] = st[i];
     if (info != 0) return fail(KSH_SOLVER_ERROR, "LDL^T: zero or non-finite pivot at column " + std::to_string(info - 1));
+    return KSH_SUCCESS;
+}
+
+int band_factor_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHost> &h, int prec)
+{
+    if (h.empty()) return KSH_SUCCESS;
+    int maxbw = 0;
+    std::vector<BandDesc> dd(h.size());
+    for (size_t i = 0; i < h.size(); ++i) {
+        dd[i] = BandDesc{h[i].fac, h[i].n, h[i].bw, h[i].info, h[i].stats, h[i].x, h[i].active, 0};
+        maxbw = std::max(maxbw, h[i].bw);
+    }
+    size_t smem = prec == KSH_FP64 ? factor_smem<double>(maxbw) : factor_smem<float>(maxbw);
+    if (smem > kMaxLds) return fail(KSH_SOLVER_ERROR, "band too wide for the one-workgroup factorization");
+    int rc;
+    if ((rc = allow_lds(smem))) return rc;
+    KSH_HIP(hipMemcpyAsync(d_descs, dd.data(), sizeof(BandDesc) * dd.size(), hipMemcpyHostToDevice, s));
+    ProfScope ps("band_ldlt_batch", s);
+    if (prec == KSH_FP64)
+        hipLaunchKernelGGL(k_band_ldlt_batched<double>, dim3((unsigned)dd.size()), dim3(kThreads), smem, s,
+                           (const BandDesc *)d_descs);
+    else
+        hipLaunchKernelGGL(k_band_ldlt_batched<float>, dim3((unsigned)dd.size()), dim3(kThreads), smem, s,
+                           (const BandDesc *)d_descs);
+    KSH_HIP(hipGetLastError());
+    return KSH_SUCCESS;
+}
+
+int refine_residual_batch(hipStream_t s, RefDesc *d_descs, const std::vector<RefDesc> &h)
+{
+    if (h.empty()) return KSH_SUCCESS;
+    KSH_HIP(hipMemcpyAsync(d_descs, h.data(), sizeof(RefDesc) * h.size(), hipMemcpyHostToDevice, s));
+    ProfScope ps("refine_residual_batch", s);
+    hipLaunchKernelGGL(k_refine_residual_batched, dim3((unsigned)h.size()), dim3(kThreads), 0, s, (const RefDesc *)d_descs);
+    KSH_HIP(hipGetLastError());
+    return KSH_SUCCESS;
+}
+
+int refine_update_batch(hipStream_t s, RefDesc *d_descs, const std::vector<RefDesc> &h)
+{
+    if (h.empty()) return KSH_SUCCESS;
+    KSH_HIP(hipMemcpyAsync(d_descs, h.data(), sizeof(RefDesc) * h.size(), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_refine_update_batched, dim3((unsigned)h.size()), dim3(kThreads), 0, s, (const RefDesc *)d_descs);
+    KSH_HIP(hipGetLastError());
+    return KSH_SUCCESS;
+}
+
+int band_solve_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHost> &h, int prec)
+{
+    if (h.empty()) return KSH_SUCCESS;
+    std::vector<BandDesc> dd(h.size());
+    for (size_t i = 0; i < h.size(); ++i) dd[i] = BandDesc{h[i].fac, h[i].n, h[i].bw, h[i].info, h[i].stats, h[i].x, h[i].active, 0};
+    KSH_HIP(hipMemcpyAsync(d_descs, dd.data(), sizeof(BandDesc) * dd.size(), hipMemcpyHostToDevice, s));
+    ProfScope ps("band_solve_batch", s);
+    if (prec == KSH_FP64)
+        hipLaunchKernelGGL(k_band_solve_batched<double>, dim3((unsigned)dd.size()), dim3(kThreads), 0, s,
+                           (const BandDesc *)d_descs);
+    else
+        hipLaunchKernelGGL(k_band_solve_batched<float>, dim3((unsigned)dd.size()), dim3(kThreads), 0, s,
+                           (const BandDesc *)d_descs);
+    KSH_HIP(hipGetLastError());
     return KSH_SUCCESS;
 }
 

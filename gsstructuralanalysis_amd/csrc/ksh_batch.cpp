@@ -1,0 +1,492 @@
+// ksh_batch.cpp -- batch engine: many independent shell problems run through the unchanged solver
+// classes (one host thread per problem) while their device calls are merged into batched launches.
+//
+// Each problem's gsALMCrisfield / gsALMRiks runs in its own host thread on a gsShellOps whose
+// executor is an engine proxy: every operator-surface call (tangent, residual, factor, solve,
+// inertia) posts a request and blocks.  When every live thread is blocked, the engine thread
+// takes the pending requests, groups them by kind and runs each group as one launch over all the
+// problems in it (element assembly over the union of all elements, one factorization workgroup
+// per band, one solve workgroup per band, batched refinement), then releases the threads.  The
+// solver logic per instance is exactly the single-problem logic; only the device work is merged.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "gs/gsALMCrisfield.h"
+#include "gs/gsALMRiks.h"
+#include "gs/ksh_ops.h"
+#include "ksh_internal.hpp"
+#include "ksh_profile.hpp"
+
+namespace ksh {
+
+int alm_driver(ksh_problem *pb, gismo::gsExec *exec, const ksh_alm_options *o, ksh_alm_result *res);
+
+namespace {
+
+enum ReqKind { RQ_TANGENT = 0, RQ_RESIDUAL, RQ_FACTOR, RQ_SOLVE, RQ_INERTIA, RQ_OTHER };
+
+struct Req {
+    ReqKind kind;
+    ksh_problem *pb = nullptr;
+    ksh_matrix *K = nullptr;
+    const double *in = nullptr;
+    double *out = nullptr;
+    double lam = 0.0;
+    int conv = 0;
+    int max_refine = 0;
+    double tol = 0.0;
+    int64_t *neg = nullptr;
+    double *dmin = nullptr;
+    int rc = 0;
+    bool done = false;
+};
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t b)
+    {
+        if (b <= bytes) return KSH_SUCCESS;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        KSH_HIP(hipMalloc(&p, b));
+        bytes = b;
+        return KSH_SUCCESS;
+    }
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+struct HostBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t b)
+    {
+        if (b <= bytes) return KSH_SUCCESS;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+        KSH_HIP(hipHostMalloc(&p, b, hipHostMallocDefault));
+        bytes = b;
+        return KSH_SUCCESS;
+    }
+    ~HostBuf()
+    {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
+class Engine {
+public:
+    explicit Engine(int nthreads) : m_active(nthreads) {}
+    ~Engine()
+    {
+        if (m_stream) (void)hipStreamDestroy(m_stream);
+    }
+    int init()
+    {
+        KSH_HIP(hipStreamCreateWithFlags(&m_stream, hipStreamNonBlocking));
+        return KSH_SUCCESS;
+    }
+
+    // instance side --------------------------------------------------------------------------
+    int submit(Req &r)
+    {
+        std::unique_lock<std::mutex> lk(m_mu);
+        m_pending.push_back(&r);
+        ++m_blocked;
+        m_cv_main.notify_one();
+        m_cv_done.wait(lk, [&] { return r.done; });
+        return r.rc;
+    }
+    void finished()
+    {
+        std::lock_guard<std::mutex> lk(m_mu);
+        --m_active;
+        m_cv_main.notify_one();
+    }
+
+    // engine side ------------------------------------------------------------------------------
+    int run()
+    {
+        int first_error = KSH_SUCCESS;
+        for (;;) {
+            std::vector<Req *> batch;
+            {
+                std::unique_lock<std::mutex> lk(m_mu);
+                m_cv_main.wait(lk, [&] { return m_active == 0 || (m_blocked == m_active && !m_pending.empty()); });
+                if (m_active == 0 && m_pending.empty()) break;
+                batch.swap(m_pending);
+            }
+            int rc = execute(batch);
+            if (rc && !first_error) first_error = rc;
+            {
+                std::lock_guard<std::mutex> lk(m_mu);
+                for (Req *r : batch) {
+                    if (rc && r->rc == KSH_SUCCESS) r->rc = rc;
+                    r->done = true;
+                }
+                m_blocked -= (int)batch.size();
+                ++m_rounds;
+            }
+            m_cv_done.notify_all();
+        }
+        return first_error;
+    }
+    long long rounds() const { return m_rounds; }
+
+private:
+    int execute(std::vector<Req *> &batch)
+    {
+        std::vector<Req *> by[6];
+        for (Req *r : batch) by[r->kind].push_back(r);
+        int rc;
+        if (!by[RQ_TANGENT].empty() && (rc = do_tangent(by[RQ_TANGENT]))) return rc;
+        if (!by[RQ_RESIDUAL].empty() && (rc = do_residual(by[RQ_RESIDUAL]))) return rc;
+        if (!by[RQ_FACTOR].empty() && (rc = do_factor(by[RQ_FACTOR]))) return rc;
+        if (!by[RQ_SOLVE].empty() && (rc = do_solve(by[RQ_SOLVE]))) return rc;
+        for (Req *r : by[RQ_INERTIA]) r->rc = ksh_inertia(r->K, r->neg, r->dmin);
+        KSH_HIP(hipStreamSynchronize(m_stream));
+        return KSH_SUCCESS;
+    }
+
+    // stage the host input vectors of the group into one pinned buffer and one device buffer
+    int stage_in(std::vector<Req *> &g, std::vector<size_t> &off, bool need_out)
+    {
+        size_t tot = 0;
+        off.resize(g.size());
+        for (size_t i = 0; i < g.size(); ++i) {
+            off[i] = tot;
+            tot += (size_t)g[i]->pb->nfree;
+        }
+        int rc;
+        if ((rc = m_hin.ensure(tot * sizeof(double)))) return rc;
+        if ((rc = m_din.ensure(tot * sizeof(double)))) return rc;
+        if (need_out) {
+            if ((rc = m_hout.ensure(tot * sizeof(double)))) return rc;
+            if ((rc = m_dout.ensure(tot * sizeof(double)))) return rc;
+        }
+        double *h = (double *)m_hin.p;
+        for (size_t i = 0; i < g.size(); ++i) std::memcpy(h + off[i], g[i]->in, sizeof(double) * g[i]->pb->nfree);
+        KSH_HIP(hipMemcpyAsync(m_din.p, m_hin.p, tot * sizeof(double), hipMemcpyHostToDevice, m_stream));
+        m_tot = tot;
+        return KSH_SUCCESS;
+    }
+
+    int ensure_desc(size_t n)
+    {
+        int rc;
+        if ((rc = m_desc.ensure(n * sizeof(InstDesc)))) return rc;
+        if ((rc = m_bdesc.ensure(n * kBandDescBytes))) return rc;
+        if ((rc = m_rdesc.ensure(n * sizeof(RefDesc)))) return rc;
+        if ((rc = m_stats.ensure(n * 4 * sizeof(double)))) return rc;
+        if ((rc = m_info.ensure(n * sizeof(int)))) return rc;
+        if ((rc = m_norms.ensure(n * 2 * sizeof(double)))) return rc;
+        return KSH_SUCCESS;
+    }
+
+    int do_tangent(std::vector<Req *> &g)
+    {
+        std::vector<size_t> off;
+        int rc;
+        if ((rc = ensure_desc(g.size())) || (rc = stage_in(g, off, false))) return rc;
+        // one launch per storage precision (normally a single group)
+        for (int sp : {KSH_FP64, KSH_FP32}) {
+            std::vector<InstDesc> h;
+            for (size_t i = 0; i < g.size(); ++i) {
+                Matrix &K = *g[i]->K;
+                if (K.sprec != sp) continue;
+                InstDesc d = make_desc(*g[i]->pb);
+                d.u_free = (const double *)m_din.p + off[i];
+                d.ab = K.ab;
+                d.sprec = K.sprec;
+                h.push_back(d);
+            }
+            if (!h.empty() && (rc = batch_tangent(m_stream, (InstDesc *)m_desc.p, h))) return rc;
+        }
+        for (size_t i = 0; i < g.size(); ++i) {
+            Matrix &K = *g[i]->K;
+            if (K.sprec == KSH_FP32) {
+                KSH_HIP(hipMemcpyAsync(K.u_at, (const double *)m_din.p + off[i], sizeof(double) * K.n, hipMemcpyDeviceToDevice,
+                                       m_stream));
+                K.has_u = true;
+            }
+            ++K.version;
+            g[i]->rc = KSH_SUCCESS;
+        }
+        return KSH_SUCCESS;
+    }
+
+    int do_residual(std::vector<Req *> &g)
+    {
+        std::vector<size_t> off;
+        int rc;
+        if ((rc = ensure_desc(g.size())) || (rc = stage_in(g, off, true))) return rc;
+        std::vector<InstDesc> h;
+        for (size_t i = 0; i < g.size(); ++i) {
+            InstDesc d = make_desc(*g[i]->pb);
+            d.u_free = (const double *)m_din.p + off[i];
+            d.lambda = g[i]->lam;
+            d.mode = (g[i]->conv == KSH_RES_ALM) ? 1 : 2;
+            d.R = (double *)m_dout.p + off[i];
+            h.push_back(d);
+        }
+        if ((rc = batch_residual(m_stream, (InstDesc *)m_desc.p, h))) return rc;
+        KSH_HIP(hipMemcpyAsync(m_hout.p, m_dout.p, m_tot * sizeof(double), hipMemcpyDeviceToHost, m_stream));
+        KSH_HIP(hipStreamSynchronize(m_stream));
+        const double *ho = (const double *)m_hout.p;
+        for (size_t i = 0; i < g.size(); ++i) {
+            int n = g[i]->pb->nfree;
+            std::memcpy(g[i]->out, ho + off[i], sizeof(double) * n);
+            g[i]->rc = KSH_SUCCESS;
+            for (int k = 0; k < n; ++k)
+                if (!std::isfinite(ho[off[i] + k])) {
+                    g[i]->rc = fail(KSH_ASSEMBLY_ERROR, "non-finite residual");
+                    break;
+                }
+        }
+        return KSH_SUCCESS;
+    }
+
+    int do_factor(std::vector<Req *> &g)
+    {
+        int rc;
+        if ((rc = ensure_desc(g.size()))) return rc;
+        std::vector<Req *> todo;
+        for (Req *r : g) {
+            Matrix &K = *r->K;
+            if (K.factored && K.factored_version == K.version) r->rc = K.info ? KSH_SOLVER_ERROR : KSH_SUCCESS;
+            else todo.push_back(r);
+        }
+        if (todo.empty()) return KSH_SUCCESS;
+        for (int fp : {KSH_FP64, KSH_FP32}) {
+            std::vector<BandDescHost> h;
+            std::vector<Req *> grp;
+            for (Req *r : todo) {
+                Matrix &K = *r->K;
+                if (K.fprec != fp) continue;
+                size_t count = (size_t)K.n * K.ld;
+                if (K.sprec == fp) {
+                    KSH_HIP(hipMemcpyAsync(K.fac, K.ab, count * (fp == KSH_FP64 ? 8 : 4), hipMemcpyDeviceToDevice, m_stream));
+                } else if ((rc = band_convert(K.ab, K.sprec, K.fac, fp, count, m_stream))) {
+                    return rc;
+                }
+                size_t k = grp.size();
+                h.push_back(BandDescHost{K.fac, K.n, K.bw, (int *)m_info.p + k, (double *)m_stats.p + 4 * k, nullptr, 1});
+                grp.push_back(r);
+            }
+            if (grp.empty()) continue;
+            if ((rc = band_factor_batch(m_stream, m_bdesc.p, h, fp))) return rc;
+            std::vector<int> info(grp.size());
+            std::vector<double> st(4 * grp.size());
+            KSH_HIP(hipMemcpyAsync(info.data(), m_info.p, sizeof(int) * grp.size(), hipMemcpyDeviceToHost, m_stream));
+            KSH_HIP(hipMemcpyAsync(st.data(), m_stats.p, sizeof(double) * 4 * grp.size(), hipMemcpyDeviceToHost, m_stream));
+            KSH_HIP(hipStreamSynchronize(m_stream));
+            for (size_t k = 0; k < grp.size(); ++k) {
+                Matrix &K = *grp[k]->K;
+                K.factored = true;
+                K.factored_version = K.version;
+                K.use64 = false;
+                K.info = info[k];
+                for (int q = 0; q < 4; ++q) K.fstats[q] = st[4 * k + q];
+                K.has_stats = info[k] == 0;
+                bool fallback = fp == KSH_FP32 && (info[k] != 0 || !(K.fstats[3] >= K.guard * K.fstats[2]));
+                if (fallback) {
+                    // the problem's own stream must see the engine's writes
+                    KSH_HIP(hipStreamSynchronize(m_stream));
+                    int r2 = refactor_fp64(K);
+                    K.info = r2 == KSH_SUCCESS ? 0 : 1;
+                }
+                grp[k]->rc = K.info ? fail(KSH_SOLVER_ERROR, "LDL^T: zero or non-finite pivot") : KSH_SUCCESS;
+            }
+        }
+        return KSH_SUCCESS;
+    }
+
+    int do_solve(std::vector<Req *> &g)
+    {
+        std::vector<size_t> off;
+        int rc;
+        if ((rc = ensure_desc(g.size())) || (rc = stage_in(g, off, true))) return rc;
+        if ((rc = m_dr.ensure(m_tot * sizeof(double)))) return rc;
+        double *b = (double *)m_din.p, *x = (double *)m_dout.p, *r = (double *)m_dr.p;
+        KSH_HIP(hipMemcpyAsync(x, b, m_tot * sizeof(double), hipMemcpyDeviceToDevice, m_stream));
+        for (Req *q : g)
+            if (!q->K->factored || q->K->factored_version != q->K->version || q->K->info)
+                return fail(KSH_SOLVER_ERROR, "matrix is not factored");
+        // solve groups by the precision of the factor in use
+        auto solve_all = [&](double *rhs_base, const std::vector<int> &active) -> int {
+            for (int fp : {KSH_FP64, KSH_FP32}) {
+                std::vector<BandDescHost> h;
+                for (size_t i = 0; i < g.size(); ++i) {
+                    Matrix &K = *g[i]->K;
+                    int efp = K.use64 ? KSH_FP64 : K.fprec;
+                    if (efp != fp) continue;
+                    void *F = K.use64 ? (void *)K.fac64 : K.fac;
+                    h.push_back(BandDescHost{F, K.n, K.bw, nullptr, nullptr, rhs_base + off[i], active[i]});
+                }
+                int rc2;
+                if (!h.empty() && (rc2 = band_solve_batch(m_stream, m_bdesc.p, h, fp))) return rc2;
+            }
+            return KSH_SUCCESS;
+        };
+        std::vector<int> active(g.size(), 1);
+        if ((rc = solve_all(x, active))) return rc;
+        int max_refine = 0;
+        double tol = 0.0;
+        for (Req *q : g) {
+            max_refine = std::max(max_refine, q->max_refine);
+            tol = std::max(tol, q->tol);
+        }
+        for (int it = 0; it < max_refine; ++it) {
+            std::vector<RefDesc> h(g.size());
+            for (size_t i = 0; i < g.size(); ++i) {
+                Matrix &K = *g[i]->K;
+                h[i] = RefDesc{K.ab, K.sprec, K.n, K.bw, active[i], b + off[i], x + off[i], r + off[i],
+                               (double *)m_norms.p + 2 * i};
+            }
+            if ((rc = refine_residual_batch(m_stream, (RefDesc *)m_rdesc.p, h))) return rc;
+            std::vector<double> nr(2 * g.size());
+            KSH_HIP(hipMemcpyAsync(nr.data(), m_norms.p, sizeof(double) * nr.size(), hipMemcpyDeviceToHost, m_stream));
+            KSH_HIP(hipStreamSynchronize(m_stream));
+            bool any = false;
+            for (size_t i = 0; i < g.size(); ++i) {
+                if (!active[i]) continue;
+                double rn = std::sqrt(nr[2 * i]), bn = std::sqrt(nr[2 * i + 1]);
+                if (!(rn > g[i]->tol * bn) || it >= g[i]->max_refine) active[i] = 0;
+                else any = true;
+            }
+            if (!any) break;
+            if ((rc = solve_all(r, active))) return rc;
+            for (size_t i = 0; i < g.size(); ++i) h[i].active = active[i];
+            if ((rc = refine_update_batch(m_stream, (RefDesc *)m_rdesc.p, h))) return rc;
+        }
+        KSH_HIP(hipMemcpyAsync(m_hout.p, x, m_tot * sizeof(double), hipMemcpyDeviceToHost, m_stream));
+        KSH_HIP(hipStreamSynchronize(m_stream));
+        const double *ho = (const double *)m_hout.p;
+        for (size_t i = 0; i < g.size(); ++i) {
+            int n = g[i]->pb->nfree;
+            std::memcpy(g[i]->out, ho + off[i], sizeof(double) * n);
+            g[i]->rc = KSH_SUCCESS;
+            for (int k = 0; k < n; ++k)
+                if (!std::isfinite(ho[off[i] + k])) {
+                    g[i]->rc = fail(KSH_SOLVER_ERROR, "non-finite solution");
+                    break;
+                }
+        }
+        return KSH_SUCCESS;
+    }
+
+    std::mutex m_mu;
+    std::condition_variable m_cv_main, m_cv_done;
+    int m_active = 0, m_blocked = 0;
+    long long m_rounds = 0;
+    std::vector<Req *> m_pending;
+    hipStream_t m_stream = nullptr;
+    DevBuf m_din, m_dout, m_dr, m_desc, m_bdesc, m_rdesc, m_stats, m_info, m_norms;
+    HostBuf m_hin, m_hout;
+    size_t m_tot = 0;
+};
+
+// executor of one instance: posts requests to the engine
+class EngineExec : public gismo::gsExec {
+public:
+    explicit EngineExec(Engine *e) : m_e(e) {}
+    int tangent(ksh_problem *pb, const double *u, ksh_matrix *K) override
+    {
+        Req r{RQ_TANGENT};
+        r.pb = pb;
+        r.K = K;
+        r.in = u;
+        return m_e->submit(r);
+    }
+    int residual(ksh_problem *pb, const double *u, double lam, int conv, double *R) override
+    {
+        Req r{RQ_RESIDUAL};
+        r.pb = pb;
+        r.in = u;
+        r.out = R;
+        r.lam = lam;
+        r.conv = conv;
+        return m_e->submit(r);
+    }
+    int factor(ksh_matrix *K) override
+    {
+        Req r{RQ_FACTOR};
+        r.pb = K->pb == nullptr ? nullptr : static_cast<ksh_problem *>(K->pb);
+        r.K = K;
+        return m_e->submit(r);
+    }
+    int solve(ksh_matrix *K, const double *b, double *x, int max_refine, double tol) override
+    {
+        Req r{RQ_SOLVE};
+        r.pb = static_cast<ksh_problem *>(K->pb);
+        r.K = K;
+        r.in = b;
+        r.out = x;
+        r.max_refine = max_refine;
+        r.tol = tol;
+        return m_e->submit(r);
+    }
+    int inertia(ksh_matrix *K, int64_t *neg, double *dmin) override
+    {
+        Req r{RQ_INERTIA};
+        r.K = K;
+        r.neg = neg;
+        r.dmin = dmin;
+        return m_e->submit(r);
+    }
+
+private:
+    Engine *m_e;
+};
+
+}  // namespace
+}  // namespace ksh
+
+extern "C" int ksh_batch_run_alm(ksh_problem *const *pbs, int count, const ksh_alm_options *opt, ksh_alm_result *res,
+                                 double *wall_seconds)
+{
+    using namespace ksh;
+    if (!pbs || count <= 0 || !opt || !res) return fail(KSH_OTHER_ERROR, "null argument");
+    for (int i = 0; i < count; ++i)
+        if (!pbs[i]) return fail(KSH_OTHER_ERROR, "null problem");
+    Engine eng(count);
+    int rc;
+    if ((rc = eng.init())) return rc;
+    std::vector<int> codes(count, KSH_SUCCESS);
+    std::vector<std::thread> threads;
+    threads.reserve(count);
+    auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < count; ++i) {
+        threads.emplace_back([&, i] {
+            EngineExec ex(&eng);
+            try {
+                codes[i] = alm_driver(pbs[i], &ex, opt, &res[i]);
+            } catch (...) {
+                codes[i] = KSH_OTHER_ERROR;
+            }
+            eng.finished();  // always: the engine waits for every live instance
+        });
+    }
+    int erc = eng.run();
+    for (auto &t : threads) t.join();
+    double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (wall_seconds) *wall_seconds = wall;
+    if (erc) return erc;
+    for (int i = 0; i < count; ++i)
+        if (codes[i] != KSH_SUCCESS && codes[i] != KSH_NOT_CONVERGED) return codes[i];
+    for (int i = 0; i < count; ++i)
+        if (codes[i] != KSH_SUCCESS) return KSH_NOT_CONVERGED;
+    return KSH_SUCCESS;
+}

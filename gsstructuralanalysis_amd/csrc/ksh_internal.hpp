@@ -20,6 +20,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/ksh.h"
 
@@ -38,6 +39,32 @@ int fail(int code, const std::string &msg);
         if (e_ != hipSuccess)                                                               \
             return ::ksh::fail(KSH_OTHER_ERROR, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
+
+// One instance of a batched launch (device-resident array; a single-instance call is a batch of
+// one).  Prefix offsets let every block of a launch over several instances find its instance.
+struct InstDesc {
+    int p, nx, ny, mx, nel, nfree, bw, ndof;
+    int elem0;              // first element of this instance in the launch
+    int dof0;               // first free dof
+    int full0;              // first full dof
+    int pad_;
+    long long slot0;        // first band slot (n*(bw+1) per instance)
+    double t, rho, lambda;
+    int mode, sprec;        // residual gather mode (0 sum, 1 ALM, 2 static); band storage precision
+    const double *ctrl0, *tabx, *taby, *gpgeo, *F, *u_free;
+    const int *free_idx, *free2full;
+    double *u_full, *Fe, *R;
+    void *Ke, *ab;
+};
+
+struct Problem;
+struct Matrix;
+
+// batched launchers over (problem, matrix / vectors) lists, all on stream s
+int batch_construct(hipStream_t s, InstDesc *d_desc, const std::vector<InstDesc> &h);
+int batch_residual(hipStream_t s, InstDesc *d_desc, const std::vector<InstDesc> &h);
+int batch_tangent(hipStream_t s, InstDesc *d_desc, const std::vector<InstDesc> &h);
+InstDesc make_desc(const Problem &pb);
 
 struct Problem {
     int p = 0, nx = 0, ny = 0, mx = 0, my = 0, nq = 0, nb = 0, ne = 0;
@@ -63,6 +90,7 @@ struct Problem {
     double *pl_N = nullptr;        // [npoint][2][p+1] basis values at the load points
     int *pl_span = nullptr;        // [npoint][2] knot spans
     double *pl_F = nullptr;        // [npoint][3] load vectors
+    InstDesc *d_desc = nullptr;    // descriptor of this problem for single-instance launches
 };
 
 struct Matrix {
@@ -82,6 +110,8 @@ struct Matrix {
     uint64_t factored_version = ~0ull;
     int info = 0;                  // 0 or 1-based index of a zero / non-finite pivot
     double guard = 1e-6;
+    double fstats[4] = {0, 0, 0, 0};  // pivot statistics of the current factor (negatives, min d, max|d|, min|d|)
+    bool has_stats = false;
     // device workspace for solves and inertia
     double *work = nullptr;        // 4*n doubles
     int *flag = nullptr;           // small device int buffer
@@ -96,7 +126,32 @@ int launch_residual(Problem &pb, double lambda, int conv, double *R_dev);
 int launch_tangent(Problem &pb, Matrix &K);
 int launch_mass(Problem &pb, Matrix &M);
 
-int band_factor(Matrix &K, void *fac, int prec);
+// stats_out (optional, 4 doubles): negatives, min d, max |d|, min |d| of the new factor
+int band_factor(Matrix &K, void *fac, int prec, double *stats_out = nullptr);
+
+// one matrix of a batched factor / solve launch; `active` = 0 skips it
+struct BandDescHost {
+    void *fac;
+    int n, bw;
+    int *info;
+    double *stats;
+    double *x;
+    int active;
+};
+int band_factor_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHost> &h, int prec);
+// batched refinement: r = b - K x with norms (||r||^2, ||b||^2), then x += r on active matrices
+struct RefDesc {
+    const void *ab;
+    int sprec, n, bw, active;
+    const double *b;
+    double *x, *r, *norms;
+};
+int refine_residual_batch(hipStream_t s, RefDesc *d_descs, const std::vector<RefDesc> &h);
+int refine_update_batch(hipStream_t s, RefDesc *d_descs, const std::vector<RefDesc> &h);
+// FP64 refactorization of K's assembled values into fac64 (K.use64 = true afterwards)
+int refactor_fp64(Matrix &K);
+int band_solve_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHost> &h, int prec);
+constexpr size_t kBandDescBytes = 48;
 int band_solve(Matrix &K, const void *fac, int prec, double *x_dev);
 int band_symv(const Matrix &K, const double *x_dev, double *y_dev);
 int band_inertia(Matrix &K, const void *fac, int prec, int64_t *neg, double *dmin, double *dmaxabs,

@@ -25,13 +25,17 @@ double now_s()
 
 }  // namespace
 
-extern "C" int ksh_run_alm(ksh_problem *pb, const ksh_alm_options *o, ksh_alm_result *res)
+namespace ksh {
+
+// One arc-length path on the operator surface of `pb`, device calls issued through `exec`
+// (direct, or an engine that merges the calls of many problems).
+int alm_driver(ksh_problem *pb, gsExec *exec, const ksh_alm_options *o, ksh_alm_result *res)
 {
     if (!pb || !o || !res) return ksh::fail(KSH_OTHER_ERROR, "null argument");
     try {
         gsSparseSolver<real_t>::settings().max_refine = o->max_refine;
         gsSparseSolver<real_t>::settings().refine_tol = o->refine_tol;
-        gsShellOps ops(pb, o->precision);
+        gsShellOps ops(pb, o->precision, exec);
         const index_t n = ops.numDofs();
         gsVector<real_t> Force = ops.force();
         auto Jacobian = ops.jacobian();
@@ -112,8 +116,22 @@ extern "C" int ksh_run_alm(ksh_problem *pb, const ksh_alm_options *o, ksh_alm_re
         res->solves = alm->solveCount();
     } catch (const std::exception &e) {
         return ksh::fail(KSH_OTHER_ERROR, e.what());
+    } catch (int code) {
+        // the solver classes signal failures outside step()/solve() (e.g. initialize()) with the
+        // reference's integer codes: 1 NotConverged, 2 AssemblyError, 3 SolverError
+        return ksh::fail(code >= 1 && code <= 3 ? code : KSH_OTHER_ERROR,
+                         std::string("solver raised status ") + std::to_string(code) + ": " + ksh_last_error());
+    } catch (...) {
+        return ksh::fail(KSH_OTHER_ERROR, "unknown exception in the solver driver");
     }
     return res->steps_done == o->nsteps ? KSH_SUCCESS : KSH_NOT_CONVERGED;
+}
+
+}  // namespace ksh
+
+extern "C" int ksh_run_alm(ksh_problem *pb, const ksh_alm_options *o, ksh_alm_result *res)
+{
+    return ksh::alm_driver(pb, gsExec::direct(), o, res);
 }
 
 extern "C" int ksh_run_static(ksh_problem *pb, const ksh_static_options *o, ksh_static_result *res)
@@ -158,6 +176,13 @@ extern "C" int ksh_run_static(ksh_problem *pb, const ksh_static_options *o, ksh_
         res->jacobians = NWT.jacobianCount();
     } catch (const std::exception &e) {
         return ksh::fail(KSH_OTHER_ERROR, e.what());
+    } catch (int code) {
+        // the solver classes signal failures outside step()/solve() (e.g. initialize()) with the
+        // reference's integer codes: 1 NotConverged, 2 AssemblyError, 3 SolverError
+        return ksh::fail(code >= 1 && code <= 3 ? code : KSH_OTHER_ERROR,
+                         std::string("solver raised status ") + std::to_string(code) + ": " + ksh_last_error());
+    } catch (...) {
+        return ksh::fail(KSH_OTHER_ERROR, "unknown exception in the solver driver");
     }
     return res->steps_done == o->nsteps ? KSH_SUCCESS : KSH_NOT_CONVERGED;
 }

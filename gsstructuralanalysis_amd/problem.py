@@ -26,6 +26,65 @@ def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+def alm_options(nsteps, length, method=0, precision=L.FP64, max_iter=20, tol=1e-6, tol_u=1e-6, tol_f=1e-3,
+                scaling=-1.0, angle_method=0, bifurcation_method=0, relaxation=1.0, quasi=0, quasi_iterations=-1,
+                adaptive=0, adaptive_iterations=10, max_refine=0, refine_tol=1e-14, max_halvings=10, verbose=0):
+    """ksh_alm_options with the reference's option names in snake case (gsALMBase.hpp:22-54)."""
+    return L.AlmOptions(method, precision, nsteps, length, max_iter, tol, tol_u, tol_f, scaling, angle_method,
+                        bifurcation_method, relaxation, quasi, quasi_iterations, adaptive, adaptive_iterations,
+                        max_refine, refine_tol, max_halvings, verbose)
+
+
+def _alm_result(nsteps, nfree, keep_U, probe_dof):
+    out = {k: np.zeros(nsteps) for k in ("lambda", "unorm", "indicator", "residue_f", "length", "w")}
+    out["negatives"] = np.zeros(nsteps, dtype=np.int64)
+    out["iterations"] = np.zeros(nsteps, dtype=np.int32)
+    U = np.zeros(nsteps * nfree) if keep_U else None
+    r = L.AlmResult()
+    r.lambda_ = _dptr(out["lambda"])
+    r.unorm = _dptr(out["unorm"])
+    r.indicator = _dptr(out["indicator"])
+    r.negatives = out["negatives"].ctypes.data_as(C.POINTER(C.c_int64))
+    r.iterations = out["iterations"].ctypes.data_as(C.POINTER(C.c_int))
+    r.residue_f = _dptr(out["residue_f"])
+    r.length = _dptr(out["length"])
+    r.w = _dptr(out["w"])
+    r.probe_dof = probe_dof
+    r.U = _dptr(U) if U is not None else None
+    return r, out, U
+
+
+def _alm_collect(r, out, U, nsteps, nfree):
+    k = r.steps_done
+    res = {key: val[:k] for key, val in out.items()}
+    res.update(steps_done=k, halvings=r.halvings, final_status=r.final_status, wall_seconds=r.wall_seconds,
+               phi=r.phi, jacobians=r.jacobians, residuals=r.residuals, solves=r.solves)
+    if U is not None:
+        res["U"] = U.reshape(nsteps, nfree)[:k]
+    return res
+
+
+def run_alm_batch(problems, nsteps, length, keep_U=False, **kw):
+    """Arc-length paths of many problems at once (ksh_batch_run_alm): each runs the unchanged
+    solver loop in its own host thread; device calls are merged into batched launches.
+    Returns (list of per-problem result dicts, wall seconds of the whole batch)."""
+    o = alm_options(nsteps, length, **kw)
+    n = len(problems)
+    handles = (C.c_void_p * n)(*[p.handle for p in problems])
+    rs = (L.AlmResult * n)()
+    keep = []
+    for i, p in enumerate(problems):
+        r, out, U = _alm_result(nsteps, p.nfree, keep_U, -1)
+        rs[i] = r
+        keep.append((out, U))
+    wall = C.c_double()
+    code = L.load().ksh_batch_run_alm(handles, n, C.byref(o), rs, C.byref(wall))
+    if code not in (L.SUCCESS, L.NOT_CONVERGED):
+        L.check(code)
+    results = [_alm_collect(rs[i], keep[i][0], keep[i][1], nsteps, p.nfree) for i, p in enumerate(problems)]
+    return results, wall.value
+
+
 class BandMatrix:
     """Device band matrix (lower band, ab[J*(bw+1)+(I-J)]) with its LDL^T factor."""
 
@@ -143,38 +202,13 @@ class ShellProblem:
         return M
 
     # -- drivers ---------------------------------------------------------------------------
-    def run_alm(self, nsteps, length, method=0, precision=L.FP64, max_iter=20, tol=1e-6, tol_u=1e-6, tol_f=1e-3,
-                scaling=-1.0, angle_method=0, bifurcation_method=0, relaxation=1.0, quasi=0, quasi_iterations=-1,
-                adaptive=0, adaptive_iterations=10, max_refine=0, refine_tol=1e-14, max_halvings=10, verbose=0,
-                keep_U=False, probe_dof=-1):
-        o = L.AlmOptions(method, precision, nsteps, length, max_iter, tol, tol_u, tol_f, scaling, angle_method,
-                         bifurcation_method, relaxation, quasi, quasi_iterations, adaptive, adaptive_iterations,
-                         max_refine, refine_tol, max_halvings, verbose)
-        out = {k: np.zeros(nsteps) for k in ("lambda", "unorm", "indicator", "residue_f", "length", "w")}
-        out["negatives"] = np.zeros(nsteps, dtype=np.int64)
-        out["iterations"] = np.zeros(nsteps, dtype=np.int32)
-        U = np.zeros(nsteps * self.nfree) if keep_U else None
-        r = L.AlmResult()
-        r.lambda_ = _dptr(out["lambda"])
-        r.unorm = _dptr(out["unorm"])
-        r.indicator = _dptr(out["indicator"])
-        r.negatives = out["negatives"].ctypes.data_as(C.POINTER(C.c_int64))
-        r.iterations = out["iterations"].ctypes.data_as(C.POINTER(C.c_int))
-        r.residue_f = _dptr(out["residue_f"])
-        r.length = _dptr(out["length"])
-        r.w = _dptr(out["w"])
-        r.probe_dof = probe_dof
-        r.U = _dptr(U) if U is not None else None
+    def run_alm(self, nsteps, length, keep_U=False, probe_dof=-1, **kw):
+        o = alm_options(nsteps, length, **kw)
+        r, out, U = _alm_result(nsteps, self.nfree, keep_U, probe_dof)
         code = L.load().ksh_run_alm(self.handle, C.byref(o), C.byref(r))
         if code not in (L.SUCCESS, L.NOT_CONVERGED):
             L.check(code)
-        k = r.steps_done
-        res = {key: val[:k] for key, val in out.items()}
-        res.update(steps_done=k, halvings=r.halvings, final_status=r.final_status, wall_seconds=r.wall_seconds,
-                   phi=r.phi, jacobians=r.jacobians, residuals=r.residuals, solves=r.solves)
-        if U is not None:
-            res["U"] = U.reshape(nsteps, self.nfree)[:k]
-        return res
+        return _alm_collect(r, out, U, nsteps, self.nfree)
 
     def run_static(self, nsteps, dlambda, precision=L.FP64, max_iter=25, tol=1e-10, max_refine=0,
                    refine_tol=1e-14, verbose=0, keep_U=False):

@@ -191,6 +191,14 @@ typedef struct {
 /* benchmarks/benchmark_Roof.cpp:320-512: options, initialize, step loop with halving. */
 int ksh_run_alm(ksh_problem *pb, const ksh_alm_options *opt, ksh_alm_result *res);
 
+/* Many independent problems (uneven sizes, same degree) each run ksh_run_alm's loop with the
+ * unchanged solver classes; their device calls are merged into batched launches (assembly over
+ * the union of all elements, one LDL^T / solve workgroup per band).  res[count]; wall_seconds
+ * is the whole batch's wall time.  No counterpart in the reference, which runs one problem per
+ * process (the closest is APALM's path-interval parallelism, src/gsALMSolvers/gsAPALM.hpp:251-430). */
+int ksh_batch_run_alm(ksh_problem *const *pbs, int count, const ksh_alm_options *opt, ksh_alm_result *res,
+                      double *wall_seconds);
+
 typedef struct {
     int precision;
     int nsteps;              /* load steps of dlambda (gsControlDisplacement::step)       */
