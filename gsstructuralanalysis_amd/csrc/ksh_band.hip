@@ -359,21 +359,6 @@ __global__ __launch_bounds__(kThreads) void k_band_solve_batched(const BandDesc 
     band_solve_body<T>((const T *)d.fac, d.n, d.bw, d.x);
 }
 
-// y = K x with K symmetric, lower band stored (FP64 accumulation).
-template <typename T>
-__global__ void k_band_symv(const T *ab, int n, int bw, const double *x, double *y)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const size_t ld = (size_t)bw + 1;
-    double acc = 0.0;
-    int dmax = min(bw, n - 1 - i);
-    for (int d = 0; d <= dmax; ++d) acc += (double)ab[(size_t)i * ld + d] * x[i + d];
-    int dmax2 = min(bw, i);
-    for (int d = 1; d <= dmax2; ++d) acc += (double)ab[(size_t)(i - d) * ld + d] * x[i - d];
-    y[i] = acc;
-}
-
 // negatives / min d / max |d| / min |d| of the factored diagonal (one workgroup)
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_band_inertia(const T *fac, int n, int bw, long long *neg, double *stats)
@@ -410,57 +395,6 @@ __global__ __launch_bounds__(kThreads) void k_band_inertia(const T *fac, int n, 
         stats[1] = smax[0];
         stats[2] = sminabs[0];
     }
-}
-
-// Refinement residual of one matrix per workgroup: r = b - K x (K symmetric, lower band of the
-// assembled values, FP64 arithmetic), norms[0] = ||r||^2, norms[1] = ||b||^2.
-template <typename T>
-__device__ void refine_residual_body(const T *ab, int n, int bw, const double *b, const double *x, double *r,
-                                     double *norms)
-{
-    __shared__ double sr[kThreads], sb[kThreads];
-    const size_t ld = (size_t)bw + 1;
-    double accr = 0.0, accb = 0.0;
-    for (int i = threadIdx.x; i < n; i += kThreads) {
-        double acc = 0.0;
-        int dmax = min(bw, n - 1 - i);
-        for (int d = 0; d <= dmax; ++d) acc += (double)ab[(size_t)i * ld + d] * x[i + d];
-        int dmax2 = min(bw, i);
-        for (int d = 1; d <= dmax2; ++d) acc += (double)ab[(size_t)(i - d) * ld + d] * x[i - d];
-        double ri = b[i] - acc;
-        r[i] = ri;
-        accr += ri * ri;
-        accb += b[i] * b[i];
-    }
-    sr[threadIdx.x] = accr;
-    sb[threadIdx.x] = accb;
-    __syncthreads();
-    for (int s = kThreads / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            sr[threadIdx.x] += sr[threadIdx.x + s];
-            sb[threadIdx.x] += sb[threadIdx.x + s];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        norms[0] = sr[0];
-        norms[1] = sb[0];
-    }
-}
-
-__global__ __launch_bounds__(kThreads) void k_refine_residual_batched(const RefDesc *descs)
-{
-    const RefDesc &d = descs[blockIdx.x];
-    if (!d.active) return;
-    if (d.sprec == KSH_FP64) refine_residual_body<double>((const double *)d.ab, d.n, d.bw, d.b, d.x, d.r, d.norms);
-    else refine_residual_body<float>((const float *)d.ab, d.n, d.bw, d.b, d.x, d.r, d.norms);
-}
-
-__global__ __launch_bounds__(kThreads) void k_refine_update_batched(const RefDesc *descs)
-{
-    const RefDesc &d = descs[blockIdx.x];
-    if (!d.active) return;
-    for (int i = threadIdx.x; i < d.n; i += kThreads) d.x[i] += d.r[i];
 }
 
 template <typename TY, typename TX>
@@ -594,25 +528,6 @@ int band_factor_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHo
     return KSH_SUCCESS;
 }
 
-int refine_residual_batch(hipStream_t s, RefDesc *d_descs, const std::vector<RefDesc> &h)
-{
-    if (h.empty()) return KSH_SUCCESS;
-    KSH_HIP(hipMemcpyAsync(d_descs, h.data(), sizeof(RefDesc) * h.size(), hipMemcpyHostToDevice, s));
-    ProfScope ps("refine_residual_batch", s);
-    hipLaunchKernelGGL(k_refine_residual_batched, dim3((unsigned)h.size()), dim3(kThreads), 0, s, (const RefDesc *)d_descs);
-    KSH_HIP(hipGetLastError());
-    return KSH_SUCCESS;
-}
-
-int refine_update_batch(hipStream_t s, RefDesc *d_descs, const std::vector<RefDesc> &h)
-{
-    if (h.empty()) return KSH_SUCCESS;
-    KSH_HIP(hipMemcpyAsync(d_descs, h.data(), sizeof(RefDesc) * h.size(), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_refine_update_batched, dim3((unsigned)h.size()), dim3(kThreads), 0, s, (const RefDesc *)d_descs);
-    KSH_HIP(hipGetLastError());
-    return KSH_SUCCESS;
-}
-
 int band_solve_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHost> &h, int prec)
 {
     if (h.empty()) return KSH_SUCCESS;
@@ -638,20 +553,6 @@ int band_solve(Matrix &K, const void *fac, int prec, double *x_dev)
         hipLaunchKernelGGL(k_band_solve<double>, dim3(1), dim3(kThreads), 0, pb.stream, (const double *)fac, K.n, K.bw, x_dev);
     else
         hipLaunchKernelGGL(k_band_solve<float>, dim3(1), dim3(kThreads), 0, pb.stream, (const float *)fac, K.n, K.bw, x_dev);
-    KSH_HIP(hipGetLastError());
-    return KSH_SUCCESS;
-}
-
-int band_symv(const Matrix &K, const double *x_dev, double *y_dev)
-{
-    Problem &pb = *K.pb;
-    ProfScope ps("band_symv", pb.stream);
-    if (K.sprec == KSH_FP64)
-        hipLaunchKernelGGL(k_band_symv<double>, dim3(nblk(K.n, 256)), dim3(256), 0, pb.stream, (const double *)K.ab, K.n, K.bw,
-                           x_dev, y_dev);
-    else
-        hipLaunchKernelGGL(k_band_symv<float>, dim3(nblk(K.n, 256)), dim3(256), 0, pb.stream, (const float *)K.ab, K.n, K.bw,
-                           x_dev, y_dev);
     KSH_HIP(hipGetLastError());
     return KSH_SUCCESS;
 }

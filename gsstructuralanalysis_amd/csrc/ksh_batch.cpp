@@ -348,12 +348,19 @@ private:
             max_refine = std::max(max_refine, q->max_refine);
             tol = std::max(tol, q->tol);
         }
+        std::vector<size_t> poff(g.size());
+        size_t ptot = 0;
+        for (size_t i = 0; i < g.size(); ++i) {
+            poff[i] = ptot;
+            ptot += 2 * (((size_t)g[i]->K->n + 63) / 64);
+        }
+        if (max_refine > 0 && (rc = m_part.ensure(ptot * sizeof(double)))) return rc;
         for (int it = 0; it < max_refine; ++it) {
             std::vector<RefDesc> h(g.size());
             for (size_t i = 0; i < g.size(); ++i) {
                 Matrix &K = *g[i]->K;
                 h[i] = RefDesc{K.ab, K.sprec, K.n, K.bw, active[i], b + off[i], x + off[i], r + off[i],
-                               (double *)m_norms.p + 2 * i};
+                               (double *)m_norms.p + 2 * i, (double *)m_part.p + poff[i], 0, 0};
             }
             if ((rc = refine_residual_batch(m_stream, (RefDesc *)m_rdesc.p, h))) return rc;
             std::vector<double> nr(2 * g.size());
@@ -393,7 +400,7 @@ private:
     long long m_rounds = 0;
     std::vector<Req *> m_pending;
     hipStream_t m_stream = nullptr;
-    DevBuf m_din, m_dout, m_dr, m_desc, m_bdesc, m_rdesc, m_stats, m_info, m_norms;
+    DevBuf m_din, m_dout, m_dr, m_desc, m_bdesc, m_rdesc, m_stats, m_info, m_norms, m_part;
     HostBuf m_hin, m_hout;
     size_t m_tot = 0;
 };

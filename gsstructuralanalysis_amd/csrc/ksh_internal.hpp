@@ -145,6 +145,8 @@ struct RefDesc {
     int sprec, n, bw, active;
     const double *b;
     double *x, *r, *norms;
+    double *part;          // 2 doubles per 64-row block (partial norms)
+    int blk0, nblk;        // set by the launcher: first row block in the launch, row blocks
 };
 int refine_residual_batch(hipStream_t s, RefDesc *d_descs, const std::vector<RefDesc> &h);
 int refine_update_batch(hipStream_t s, RefDesc *d_descs, const std::vector<RefDesc> &h);
