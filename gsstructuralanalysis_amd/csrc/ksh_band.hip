@@ -36,22 +36,11 @@ constexpr int kThreads = 256;
 
 __host__ __device__ inline int panel_rows(int bw) { return ((bw + 15) / 16) * 16 + 32; }
 
-// One matrix of a batched LDL^T / solve launch (blockIdx.x = matrix).
-struct BandDesc {
-    void *fac;       // factor storage (in place)
-    int n, bw;
-    int *info;       // 0 or 1-based zero-pivot column
-    double *stats;   // [4]: negatives, min d, max |d|, min |d|
-    double *x;       // solve: right-hand side in, solution out
-    int active;      // 0: skip this matrix
-    int pad_;
-};
-
 template <typename T>
 __device__ void pivot_stats(const T *fac, int n, int bw, double *stats);
 
 template <typename T>
-__device__ void band_ldlt_body(T *fac, int n, int bw, int *info, double *stats, unsigned char *smem)
+__device__ __forceinline__ void band_ldlt_body(T *fac, int n, int bw, int *info, double *stats, unsigned char *smem)
 {
     const int PR = panel_rows(bw);
     T *pan = (T *)smem;                     // [16][PR]
@@ -223,141 +212,6 @@ __global__ __launch_bounds__(kThreads) void k_band_ldlt_batched(const BandDesc *
     band_ldlt_body<T>((T *)d.fac, d.n, d.bw, d.info, d.stats, smem);
 }
 
-// Forward / diagonal / backward substitution with the factored band (one workgroup per matrix).
-// Blocks of SB = 64 rows.  Forward block: every thread accumulates part of the left-looking
-// product sum_{j<r0} L[i][j] y[j] for the block's rows with coalesced loads (lanes along i within
-// a band column, 16 loads issued before use), the 64x64 diagonal block is staged through LDS into
-// registers (lane = row), then wave 0 finishes the block with a 64-step recurrence whose
-// broadcasts are v_readlane (no LDS on the dependency chain).  Backward block: each wave reduces
-// 16 columns' dot products sum_{i>=c0+64} L[i][j] x[i] with lanes along i, then wave 0 runs the
-// transposed recurrence.  The band is read exactly twice.
-constexpr int SB = 64;
-constexpr int SU = 16;  // loads in flight per thread in the forward product
-
-__device__ inline double readlane_f64(double v, int lane)
-{
-    int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-    int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-    return __hiloint2double(hi, lo);
-}
-
-template <typename T>
-__device__ void band_solve_body(const T *__restrict__ fac, int n, int bw, double *__restrict__ x)
-{
-    __shared__ double sdiag[SB][SB + 1];  // sdiag[jj][ii] = L[r0+ii][r0+jj]
-    __shared__ double spart[4][SB];
-    const size_t ld = (size_t)bw + 1;
-    const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
-    // ---- forward: L y = b ----
-    for (int r0 = 0; r0 < n; r0 += SB) {
-        const int w = min(SB, n - r0);
-        for (int idx = tid; idx < SB * SB; idx += kThreads) {
-            int jj = idx / SB, ii = idx % SB;
-            double v = 0.0;
-            if (jj < w && ii < w && ii > jj && ii - jj <= bw) v = (double)fac[(size_t)(r0 + jj) * ld + (ii - jj)];
-            sdiag[jj][ii] = v;
-        }
-        {
-            const int i = r0 + lane;
-            const int jlo = max(0, i - bw);
-            double acc = 0.0;
-            const int jbeg = max(0, r0 - bw) + wave;
-            for (int base = jbeg; base < r0; base += 4 * SU) {
-                double lv[SU], xv[SU];
-#pragma unroll
-                for (int u = 0; u < SU; ++u) {
-                    int j = base + 4 * u;
-                    bool ok = lane < w && j < r0 && j >= jlo;
-                    lv[u] = ok ? (double)fac[(size_t)j * ld + (i - j)] : 0.0;
-                    xv[u] = (j < r0) ? x[j] : 0.0;
-                }
-#pragma unroll
-                for (int u = 0; u < SU; ++u) acc += lv[u] * xv[u];
-            }
-            spart[wave][lane] = acc;
-        }
-        __syncthreads();
-        if (wave == 0) {
-            double row[SB];
-#pragma unroll
-            for (int jj = 0; jj < SB; ++jj) row[jj] = sdiag[jj][lane];
-            double y = 0.0;
-            if (lane < w) y = x[r0 + lane] - (spart[0][lane] + spart[1][lane] + spart[2][lane] + spart[3][lane]);
-#pragma unroll
-            for (int jj = 0; jj < SB; ++jj) {
-                double yj = readlane_f64(y, jj);
-                if (lane > jj) y -= row[jj] * yj;
-            }
-            if (lane < w) x[r0 + lane] = y;
-        }
-        __syncthreads();
-    }
-    // ---- diagonal ----
-    for (int i = tid; i < n; i += kThreads) x[i] /= (double)fac[(size_t)i * ld];
-    __syncthreads();
-    // ---- backward: L^T x = y ----
-    const int nblk = (n + SB - 1) / SB;
-    for (int bk = nblk - 1; bk >= 0; --bk) {
-        const int c0 = bk * SB, w = min(SB, n - c0);
-        for (int idx = tid; idx < SB * SB; idx += kThreads) {
-            int jj = idx / SB, ii = idx % SB;
-            double v = 0.0;
-            if (jj < w && ii < w && ii > jj && ii - jj <= bw) v = (double)fac[(size_t)(c0 + jj) * ld + (ii - jj)];
-            sdiag[jj][ii] = v;
-        }
-        double acc[16];
-#pragma unroll
-        for (int c = 0; c < 16; ++c) acc[c] = 0.0;
-        const int ibase = c0 + w;
-        const int iend = min(n, c0 + w + bw);
-        for (int i = ibase + lane; i < iend; i += 64) {
-            double xi = x[i];
-            double lv[16];
-#pragma unroll
-            for (int c = 0; c < 16; ++c) {
-                int jj = 16 * wave + c;
-                int j = c0 + jj;
-                lv[c] = (jj < w && i - j <= bw) ? (double)fac[(size_t)j * ld + (i - j)] : 0.0;
-            }
-#pragma unroll
-            for (int c = 0; c < 16; ++c) acc[c] += lv[c] * xi;
-        }
-#pragma unroll
-        for (int c = 0; c < 16; ++c) {
-            double v = acc[c];
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-            if (lane == 0) spart[0][16 * wave + c] = v;
-        }
-        __syncthreads();
-        if (wave == 0) {
-            double col[SB];  // col[jj] = L[c0+jj][c0+lane]
-#pragma unroll
-            for (int jj = 0; jj < SB; ++jj) col[jj] = sdiag[lane][jj];
-            double xv = (lane < w) ? x[c0 + lane] - spart[0][lane] : 0.0;
-#pragma unroll
-            for (int jj = SB - 1; jj >= 0; --jj) {
-                double xj = readlane_f64(xv, jj);
-                if (lane < jj) xv -= col[jj] * xj;
-            }
-            if (lane < w) x[c0 + lane] = xv;
-        }
-        __syncthreads();
-    }
-}
-
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_band_solve(const T *fac, int n, int bw, double *x)
-{
-    band_solve_body<T>(fac, n, bw, x);
-}
-
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_band_solve_batched(const BandDesc *descs)
-{
-    const BandDesc &d = descs[blockIdx.x];
-    if (!d.active) return;
-    band_solve_body<T>((const T *)d.fac, d.n, d.bw, d.x);
-}
 
 // negatives / min d / max |d| / min |d| of the factored diagonal (one workgroup)
 template <typename T>
@@ -524,35 +378,6 @@ int band_factor_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHo
     else
         hipLaunchKernelGGL(k_band_ldlt_batched<float>, dim3((unsigned)dd.size()), dim3(kThreads), smem, s,
                            (const BandDesc *)d_descs);
-    KSH_HIP(hipGetLastError());
-    return KSH_SUCCESS;
-}
-
-int band_solve_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHost> &h, int prec)
-{
-    if (h.empty()) return KSH_SUCCESS;
-    std::vector<BandDesc> dd(h.size());
-    for (size_t i = 0; i < h.size(); ++i) dd[i] = BandDesc{h[i].fac, h[i].n, h[i].bw, h[i].info, h[i].stats, h[i].x, h[i].active, 0};
-    KSH_HIP(hipMemcpyAsync(d_descs, dd.data(), sizeof(BandDesc) * dd.size(), hipMemcpyHostToDevice, s));
-    ProfScope ps("band_solve_batch", s);
-    if (prec == KSH_FP64)
-        hipLaunchKernelGGL(k_band_solve_batched<double>, dim3((unsigned)dd.size()), dim3(kThreads), 0, s,
-                           (const BandDesc *)d_descs);
-    else
-        hipLaunchKernelGGL(k_band_solve_batched<float>, dim3((unsigned)dd.size()), dim3(kThreads), 0, s,
-                           (const BandDesc *)d_descs);
-    KSH_HIP(hipGetLastError());
-    return KSH_SUCCESS;
-}
-
-int band_solve(Matrix &K, const void *fac, int prec, double *x_dev)
-{
-    Problem &pb = *K.pb;
-    ProfScope ps("band_solve", pb.stream);
-    if (prec == KSH_FP64)
-        hipLaunchKernelGGL(k_band_solve<double>, dim3(1), dim3(kThreads), 0, pb.stream, (const double *)fac, K.n, K.bw, x_dev);
-    else
-        hipLaunchKernelGGL(k_band_solve<float>, dim3(1), dim3(kThreads), 0, pb.stream, (const float *)fac, K.n, K.bw, x_dev);
     KSH_HIP(hipGetLastError());
     return KSH_SUCCESS;
 }

@@ -138,6 +138,16 @@ struct BandDescHost {
     double *x;
     int active;
 };
+// device-side form of BandDescHost (blockIdx.x of a batched factor / solve launch = matrix)
+struct BandDesc {
+    void *fac;       // factor storage (in place)
+    int n, bw;
+    int *info;       // 0 or 1-based zero-pivot column
+    double *stats;   // [4]: negatives, min d, max |d|, min |d|
+    double *x;       // solve: right-hand side in, solution out
+    int active;      // 0: skip this matrix
+    int pad_;
+};
 int band_factor_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHost> &h, int prec);
 // batched refinement: r = b - K x with norms (||r||^2, ||b||^2), then x += r on active matrices
 struct RefDesc {

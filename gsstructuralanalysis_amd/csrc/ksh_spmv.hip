@@ -25,7 +25,7 @@ constexpr int kT = 256;
 constexpr int kU = 8;
 
 template <typename T>
-__device__ double symv_row_block(const T *__restrict__ ab, int n, int bw, const double *__restrict__ x, int i0)
+__device__ __forceinline__ double symv_row_block(const T *__restrict__ ab, int n, int bw, const double *__restrict__ x, int i0)
 {
     __shared__ double part[4][kRows];
     __shared__ double up[kRows];
