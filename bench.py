@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Equilibrium steps/s of the shell arc-length path on one MI355X.
 
-Default workload ("C2", the batched form of BASELINE configs[2]): 256 shallow cylindrical panels
+Default workload ("C2", the batched arc-length study): 256 shallow cylindrical panels
 (R = 2.5, half-angle 0.1 rad, length 0.5, t = 0.0127, E = 3.1e3, nu = 0.3, degree 3), mesh
 n x n with n drawn from {16,24,32,40,48}, each mid-surface perturbed by a seeded Gaussian random
 field (correlation length 0.1, amplitude U[0, 0.5] t, PCG64(20261202 + i)); straight edges hinged,
@@ -16,7 +16,7 @@ runs one instance at a time; it is timed on one instance per mesh size and scale
 histogram of the batch.  cpu_baseline: the FP64 CPU oracle port of the same loop (reference
 re-factorisations kept) on a bounded prefix of one instance's path, on this host's cores.
 
---workload c1: the single 32x32 perfect panel of configs[1], plain vs mixed, 80 steps.
+--workload c1: the single 32x32 perfect panel (the roof benchmark refined), plain vs mixed, 80 steps.
 """
 from __future__ import annotations
 
@@ -130,8 +130,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU only (every box has one MI355X); "
-              "the multi-GPU sharding is described in DESIGN.md, not run.", file=sys.stderr)
+        print("bench.py: measures one GPU only; instances of a batch would be sharded over GPUs "
+              "one process per GPU, which is not run here.", file=sys.stderr)
         sys.exit(2)
 
     from gsstructuralanalysis_amd import _lib as L

@@ -25,48 +25,13 @@
 
 #include <algorithm>
 
+#include "ksh_devutil.hpp"
 #include "ksh_internal.hpp"
 #include "ksh_profile.hpp"
 
 namespace ksh {
 
 namespace {
-
-__device__ inline double readlane_f64(double v, int lane)
-{
-    int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-    int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-    return __hiloint2double(hi, lo);
-}
-
-// workgroup barrier that orders LDS only: the prefetched global loads in flight are private to
-// their thread and must not be waited for here
-__device__ inline void lds_barrier()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
-// Band reads go through a buffer resource covering exactly the band (n * ld * sizeof(T) bytes,
-// < 4 GiB, checked by the launcher).  A masked-off read is sent to offset 0x80000000, beyond the
-// resource, and the buffer unit returns 0 for it: masked loads stay straight-line code with many
-// loads in flight, no branch or mask is kept alive across the load, and the product with a zero
-// entry is exact.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t band_rsrc(const void *base, unsigned bytes)
-{
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
-}
-
-template <typename T>
-__device__ __forceinline__ T load_if(bool ok, __amdgpu_buffer_rsrc_t rs, unsigned byte_off)
-{
-    const int off = (int)(ok ? byte_off : 0x80000000u);
-    if constexpr (sizeof(T) == 8)
-        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
-    else
-        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
-}
 
 constexpr int kRB = 64;       // rows per block
 constexpr int kSdLd = kRB + 1;
