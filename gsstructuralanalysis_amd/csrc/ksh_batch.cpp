@@ -160,8 +160,10 @@ private:
         return KSH_SUCCESS;
     }
 
-    // stage the host input vectors of the group into one pinned buffer and one device buffer
-    int stage_in(std::vector<Req *> &g, std::vector<size_t> &off, bool need_out)
+    // Stage the host input vectors of the group into one pinned buffer and one device buffer.
+    // Each request kind has its own pair: a kind's host-to-device copy may still be queued when
+    // the next kind of the same round stages its inputs (a round ends with a stream sync).
+    int stage_in(std::vector<Req *> &g, std::vector<size_t> &off, bool need_out, int slot)
     {
         size_t tot = 0;
         off.resize(g.size());
@@ -170,15 +172,15 @@ private:
             tot += (size_t)g[i]->pb->nfree;
         }
         int rc;
-        if ((rc = m_hin.ensure(tot * sizeof(double)))) return rc;
-        if ((rc = m_din.ensure(tot * sizeof(double)))) return rc;
+        if ((rc = m_hin[slot].ensure(tot * sizeof(double)))) return rc;
+        if ((rc = m_din[slot].ensure(tot * sizeof(double)))) return rc;
         if (need_out) {
             if ((rc = m_hout.ensure(tot * sizeof(double)))) return rc;
             if ((rc = m_dout.ensure(tot * sizeof(double)))) return rc;
         }
-        double *h = (double *)m_hin.p;
+        double *h = (double *)m_hin[slot].p;
         for (size_t i = 0; i < g.size(); ++i) std::memcpy(h + off[i], g[i]->in, sizeof(double) * g[i]->pb->nfree);
-        KSH_HIP(hipMemcpyAsync(m_din.p, m_hin.p, tot * sizeof(double), hipMemcpyHostToDevice, m_stream));
+        KSH_HIP(hipMemcpyAsync(m_din[slot].p, m_hin[slot].p, tot * sizeof(double), hipMemcpyHostToDevice, m_stream));
         m_tot = tot;
         return KSH_SUCCESS;
     }
@@ -199,7 +201,8 @@ private:
     {
         std::vector<size_t> off;
         int rc;
-        if ((rc = ensure_desc(g.size())) || (rc = stage_in(g, off, false))) return rc;
+        if ((rc = ensure_desc(g.size())) || (rc = stage_in(g, off, false, 0))) return rc;
+        const double *din = (const double *)m_din[0].p;
         // one launch per storage precision (normally a single group)
         for (int sp : {KSH_FP64, KSH_FP32}) {
             std::vector<InstDesc> h;
@@ -207,7 +210,7 @@ private:
                 Matrix &K = *g[i]->K;
                 if (K.sprec != sp) continue;
                 InstDesc d = make_desc(*g[i]->pb);
-                d.u_free = (const double *)m_din.p + off[i];
+                d.u_free = din + off[i];
                 d.ab = K.ab;
                 d.sprec = K.sprec;
                 h.push_back(d);
@@ -217,7 +220,7 @@ private:
         for (size_t i = 0; i < g.size(); ++i) {
             Matrix &K = *g[i]->K;
             if (K.sprec == KSH_FP32) {
-                KSH_HIP(hipMemcpyAsync(K.u_at, (const double *)m_din.p + off[i], sizeof(double) * K.n, hipMemcpyDeviceToDevice,
+                KSH_HIP(hipMemcpyAsync(K.u_at, din + off[i], sizeof(double) * K.n, hipMemcpyDeviceToDevice,
                                        m_stream));
                 K.has_u = true;
             }
@@ -231,11 +234,11 @@ private:
     {
         std::vector<size_t> off;
         int rc;
-        if ((rc = ensure_desc(g.size())) || (rc = stage_in(g, off, true))) return rc;
+        if ((rc = ensure_desc(g.size())) || (rc = stage_in(g, off, true, 1))) return rc;
         std::vector<InstDesc> h;
         for (size_t i = 0; i < g.size(); ++i) {
             InstDesc d = make_desc(*g[i]->pb);
-            d.u_free = (const double *)m_din.p + off[i];
+            d.u_free = (const double *)m_din[1].p + off[i];
             d.lambda = g[i]->lam;
             d.mode = (g[i]->conv == KSH_RES_ALM) ? 1 : 2;
             d.R = (double *)m_dout.p + off[i];
@@ -317,9 +320,9 @@ private:
     {
         std::vector<size_t> off;
         int rc;
-        if ((rc = ensure_desc(g.size())) || (rc = stage_in(g, off, true))) return rc;
+        if ((rc = ensure_desc(g.size())) || (rc = stage_in(g, off, true, 2))) return rc;
         if ((rc = m_dr.ensure(m_tot * sizeof(double)))) return rc;
-        double *b = (double *)m_din.p, *x = (double *)m_dout.p, *r = (double *)m_dr.p;
+        double *b = (double *)m_din[2].p, *x = (double *)m_dout.p, *r = (double *)m_dr.p;
         KSH_HIP(hipMemcpyAsync(x, b, m_tot * sizeof(double), hipMemcpyDeviceToDevice, m_stream));
         for (Req *q : g)
             if (!q->K->factored || q->K->factored_version != q->K->version || q->K->info)
@@ -400,8 +403,8 @@ private:
     long long m_rounds = 0;
     std::vector<Req *> m_pending;
     hipStream_t m_stream = nullptr;
-    DevBuf m_din, m_dout, m_dr, m_desc, m_bdesc, m_rdesc, m_stats, m_info, m_norms, m_part;
-    HostBuf m_hin, m_hout;
+    DevBuf m_din[3], m_dout, m_dr, m_desc, m_bdesc, m_rdesc, m_stats, m_info, m_norms, m_part;
+    HostBuf m_hin[3], m_hout;
     size_t m_tot = 0;
 };
 

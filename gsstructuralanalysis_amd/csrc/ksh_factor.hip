@@ -245,25 +245,33 @@ __device__ __forceinline__ void band_ldlt_wg(T *fac, int n, int bw, int *info, d
                 }
                 return x;
             };
-            acc_t cur = tile_load(wave);
-            for (int p = wave; p < npair; p += kFW) {
-                const acc_t nxt = tile_load(p + kFW);  // past the last tile every read is masked off
-                int ri0, rj0;
-                decode(p, ri0, rj0);
+            // a wave keeps the A loads of its next kDepth tiles in flight
+            constexpr int kDepth = 4;
+            acc_t buf[kDepth];
 #pragma unroll
-                for (int s = 0; s < NB / 4; ++s) {
-                    const int k = 4 * s + lk;
-                    const T wv = -pan[k * PR + rj0 + li] * dv[k];
-                    const T lv = pan[k * PR + ri0 + li];
-                    cur = M::run(wv, lv, cur);
-                }
+            for (int u = 0; u < kDepth; ++u) buf[u] = tile_load(wave + kFW * u);
+            for (int p0 = wave; p0 < npair; p0 += kFW * kDepth) {
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    bool ok;
-                    const unsigned off = tile_io(ri0, rj0, v, ok);
-                    store_if<T>(ok, rs, off, cur[v]);
+                for (int u = 0; u < kDepth; ++u) {
+                    const int p = p0 + kFW * u;
+                    acc_t cur = buf[u];
+                    buf[u] = tile_load(p + kFW * kDepth);  // past the last tile every read is masked off
+                    int ri0, rj0;
+                    decode(p, ri0, rj0);
+#pragma unroll
+                    for (int s = 0; s < NB / 4; ++s) {
+                        const int k = 4 * s + lk;
+                        const T wv = -pan[k * PR + rj0 + li] * dv[k];
+                        const T lv = pan[k * PR + ri0 + li];
+                        cur = M::run(wv, lv, cur);
+                    }
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        bool ok;
+                        const unsigned off = tile_io(ri0, rj0, v, ok);
+                        store_if<T>(ok && p < npair, rs, off, cur[v]);
+                    }
                 }
-                cur = nxt;
             }
         }
         __syncthreads();  // trailing stores visible to the next panel's staging loads

@@ -33,6 +33,30 @@ def test_batch_equals_single_runs(precision, ksh):
         assert np.abs(b["lambda"] - s["lambda"]).max() <= tol * np.abs(s["lambda"]).max()
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_equals_single_runs_desynchronised(precision, ksh):
+    """Instances of different sizes take different iteration counts, so one engine round mixes
+    request kinds (tangent, residual, factor, solve of different instances); each instance must
+    still reproduce its single-problem path exactly (the C2 study's settings on smaller meshes)."""
+    from gsstructuralanalysis_amd.problem import ShellProblem, run_alm_batch
+    pts = [g.imperfect_roof_instance(i, sizes=(6, 8, 10, 12)) for i in range(12)]
+    print(f"[case] desynchronised batch of {len(pts)} imperfect roofs, precision {precision}", flush=True)
+    extra = dict(max_refine=6, refine_tol=1e-11) if precision == 1 else {}
+    sps = [ShellProblem(pt) for pt in pts]
+    batch, _ = run_alm_batch(sps, 20, 6e-3, precision=precision, keep_U=True, **KW, **extra)
+    its = set()
+    for sp, b in zip(sps, batch):
+        s = sp.run_alm(20, 6e-3, precision=precision, keep_U=True, **KW, **extra)
+        assert b["steps_done"] == s["steps_done"] == 20
+        assert np.array_equal(b["negatives"], s["negatives"])
+        assert np.array_equal(b["iterations"], s["iterations"])
+        tol = 1e-12 if precision == 0 else 1e-9
+        assert np.abs(b["U"] - s["U"]).max() <= tol * np.abs(s["U"]).max()
+        assert np.abs(b["lambda"] - s["lambda"]).max() <= tol * np.abs(s["lambda"]).max()
+        its.add(int(np.sum(s["iterations"])))
+    assert len(its) > 1  # the instances really are out of step
+
+
 def test_batch_with_failures_and_uneven_work(ksh):
     """One instance cannot converge in 2 iterations and halves its arc length; the others proceed."""
     from gsstructuralanalysis_amd.problem import ShellProblem, run_alm_batch
