@@ -69,4 +69,24 @@ __device__ __forceinline__ void store_if(bool ok, __amdgpu_buffer_rsrc_t rs, uns
         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rs, off, 0, 0);
 }
 
+// buffer access at a per-lane offset (kMaskedOff for a masked-off lane) plus a wave-uniform
+// offset carried in an SGPR
+template <typename T>
+__device__ __forceinline__ T load_vs(__amdgpu_buffer_rsrc_t rs, unsigned voff, unsigned soff)
+{
+    if constexpr (sizeof(T) == 8)
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)voff, (int)soff, 0));
+    else
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, (int)voff, (int)soff, 0));
+}
+
+template <typename T>
+__device__ __forceinline__ void store_vs(__amdgpu_buffer_rsrc_t rs, unsigned voff, unsigned soff, T v)
+{
+    if constexpr (sizeof(T) == 8)
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(ksh_u32x2, v), rs, (int)voff, (int)soff, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rs, (int)voff, (int)soff, 0);
+}
+
 }  // namespace ksh

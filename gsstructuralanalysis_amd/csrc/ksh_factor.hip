@@ -43,7 +43,9 @@ namespace {
 constexpr int kFT = 1024;            // threads per workgroup
 constexpr int kFW = kFT / 64;        // waves
 
-__host__ __device__ inline int fpanel_rows(int bw, int nb) { return ((bw + nb + 15) / 16) * 16 + 16; }
+// panel rows in LDS: the frame (nb + bw rows) rounded up, plus the rows a 32-row block past the
+// frame edge reads (their products only reach masked-off outputs)
+__host__ __device__ inline int fpanel_rows(int bw, int nb) { return ((bw + nb + 15) / 16) * 16 + 32; }
 
 template <typename T, int NB>
 __host__ __device__ inline size_t factor_lds_bytes(int bw)
@@ -107,6 +109,11 @@ __device__ __forceinline__ void band_ldlt_wg(T *fac, int n, int bw, int *info, d
     const unsigned ldu = (unsigned)bw + 1;
     const __amdgpu_buffer_rsrc_t rs = band_rsrc(fac, (unsigned)((size_t)n * ldu * sizeof(T)));
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // per-lane part of a trailing tile's band offsets: entry (ri0+li, rj0+row(lane, v)) of the
+    // frame at column c0 sits at (c0 ld + rj0 (ld-1) + ri0) + (row (ld-1) + li)
+    unsigned lane_off[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) lane_off[v] = ((unsigned)M::row(lane, v) * (ldu - 1) + (unsigned)(lane & 15)) * (unsigned)sizeof(T);
     if (tid == 0) s_info = 0;
     __syncthreads();
 
@@ -214,64 +221,84 @@ __device__ __forceinline__ void band_ldlt_wg(T *fac, int n, int bw, int *info, d
             }
         }
         // ---- 5. trailing update of rows/columns [w, nr) of the panel frame ----
+        // The lower triangle of the window is cut into 32x32 blocks (bi, bj), bj <= bi, each a 2x2
+        // set of 16x16 MFMA tiles sharing operands: A (-d_k L[j][k], rows j of block column bj)
+        // and B (L[i][k], rows i of block row bi) come from the LDS panel.  Blocks are dealt to
+        // the waves round-robin; a wave loads the A-matrix entries of its next block while it
+        // computes the current one.  Tile addresses are a per-lane part fixed for the whole
+        // factorization plus a wave-uniform part (SGPR); masked lanes and tiles (edges, the
+        // upper tile of a diagonal block, blocks past the wave's last) get kMaskedOff, so the
+        // loop is straight-line code.
         {
-            const int nt = (nr - w + 15) / 16;
-            const int npair = nt * (nt + 1) / 2;
+            const int nbk = (nr - w + 31) / 32;
+            const int nblock = nbk * (nbk + 1) / 2;
+            const int wv = __builtin_amdgcn_readfirstlane(wave);
             const int li = lane & 15, lk = lane >> 4;
-            // tile p -> (ti, tj), tj <= ti, row-major over the lower triangle of tiles
-            auto decode = [&](int p, int &ri0, int &rj0) {
-                int ti = (int)((sqrtf(8.0f * (float)p + 1.0f) - 1.0f) * 0.5f);
-                if ((ti + 1) * (ti + 2) / 2 <= p) ++ti;
-                if (ti * (ti + 1) / 2 > p) --ti;
-                const int tj = p - ti * (ti + 1) / 2;
-                ri0 = w + 16 * ti;
-                rj0 = w + 16 * tj;
+            constexpr unsigned es = sizeof(T);
+            auto decode = [&](int p, int &bi, int &bj) {
+                int b = (int)((sqrtf(8.0f * (float)p + 1.0f) - 1.0f) * 0.5f);
+                if ((b + 1) * (b + 2) / 2 <= p) ++b;
+                if (b * (b + 1) / 2 > p) --b;
+                bi = b;
+                bj = p - b * (b + 1) / 2;
             };
             // accumulator rows = j (A's column), columns = i (A's row): lanes run along a column
-            auto tile_io = [&](int ri0, int rj0, int v, bool &ok) -> unsigned {
+            auto tile_off = [&](int p, int bi, int bj, int a, int b, int v, unsigned &soff) -> unsigned {
+                const int ri0 = w + 32 * bi + 16 * a, rj0 = w + 32 * bj + 16 * b;
+                soff = ((unsigned)c0 * ldu + (unsigned)rj0 * (ldu - 1) + (unsigned)ri0) * es;
+                const bool edge = bi == bj || ri0 + 16 > nr;  // wave-uniform
                 const int i = ri0 + li, j = rj0 + M::row(lane, v);
-                ok = i < nr && j < nr && i >= j;
-                return ((unsigned)(c0 + j) * ldu + (unsigned)(i - j)) * (unsigned)sizeof(T);
+                const bool ok = p < nblock && (!edge || (i < nr && i >= j));
+                return ok ? lane_off[v] : kMaskedOff;
             };
-            auto tile_load = [&](int p) -> acc_t {
-                int ri0, rj0;
-                decode(p, ri0, rj0);
-                acc_t x;
+            auto block_load = [&](int p, acc_t (&x)[2][2]) {
+                int bi, bj;
+                decode(p, bi, bj);
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    bool ok;
-                    const unsigned off = tile_io(ri0, rj0, v, ok);
-                    x[v] = load_if<T>(ok && p < npair, rs, off);
-                }
-                return x;
+                for (int a = 0; a < 2; ++a)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            unsigned so;
+                            const unsigned vo = tile_off(p, bi, bj, a, b, v, so);
+                            x[a][b][v] = load_vs<T>(rs, vo, so);
+                        }
             };
-            // a wave keeps the A loads of its next kDepth tiles in flight
-            constexpr int kDepth = 4;
-            acc_t buf[kDepth];
+            acc_t nxt[2][2];
+            block_load(wv, nxt);
+            for (int p = wv; p < nblock; p += kFW) {
+                acc_t cur[2][2];
 #pragma unroll
-            for (int u = 0; u < kDepth; ++u) buf[u] = tile_load(wave + kFW * u);
-            for (int p0 = wave; p0 < npair; p0 += kFW * kDepth) {
+                for (int a = 0; a < 2; ++a)
 #pragma unroll
-                for (int u = 0; u < kDepth; ++u) {
-                    const int p = p0 + kFW * u;
-                    acc_t cur = buf[u];
-                    buf[u] = tile_load(p + kFW * kDepth);  // past the last tile every read is masked off
-                    int ri0, rj0;
-                    decode(p, ri0, rj0);
+                    for (int b = 0; b < 2; ++b) cur[a][b] = nxt[a][b];
+                block_load(p + kFW, nxt);  // past the last block every read is masked off
+                int bi, bj;
+                decode(p, bi, bj);
+                const int ri0 = w + 32 * bi + li, rj0 = w + 32 * bj + li;
+                const bool diag = bi == bj;  // its tile (a=0, b=1) lies above the diagonal
 #pragma unroll
-                    for (int s = 0; s < NB / 4; ++s) {
-                        const int k = 4 * s + lk;
-                        const T wv = -pan[k * PR + rj0 + li] * dv[k];
-                        const T lv = pan[k * PR + ri0 + li];
-                        cur = M::run(wv, lv, cur);
-                    }
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        bool ok;
-                        const unsigned off = tile_io(ri0, rj0, v, ok);
-                        store_if<T>(ok && p < npair, rs, off, cur[v]);
-                    }
+                for (int s = 0; s < NB / 4; ++s) {
+                    const int k = 4 * s + lk;
+                    const T nd = -dv[k];
+                    const T a0 = nd * pan[k * PR + rj0], a1 = nd * pan[k * PR + rj0 + 16];
+                    const T b0 = pan[k * PR + ri0], b1 = pan[k * PR + ri0 + 16];
+                    cur[0][0] = M::run(a0, b0, cur[0][0]);
+                    cur[1][0] = M::run(a0, b1, cur[1][0]);
+                    cur[1][1] = M::run(a1, b1, cur[1][1]);
+                    if (!diag) cur[0][1] = M::run(a1, b0, cur[0][1]);
                 }
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            unsigned so;
+                            const unsigned vo = tile_off(p, bi, bj, a, b, v, so);
+                            store_vs<T>(rs, vo, so, cur[a][b][v]);
+                        }
             }
         }
         __syncthreads();  // trailing stores visible to the next panel's staging loads
@@ -350,8 +377,8 @@ int launch_factor(hipStream_t s, int nb, unsigned grid, const BandDesc *d_descs,
 int check_factor_shape(int n, int bw, int prec, int &nb)
 {
     const size_t es = prec == KSH_FP64 ? sizeof(double) : sizeof(float);
-    if ((size_t)n * (size_t)(bw + 1) * es >= (size_t)kMaskedOff)
-        return fail(KSH_SOLVER_ERROR, "band larger than 2 GiB: not supported by the one-workgroup factorization");
+    if ((size_t)n * (size_t)(bw + 1) * es >= ((size_t)1 << 30))
+        return fail(KSH_SOLVER_ERROR, "band larger than 1 GiB: not supported by the one-workgroup factorization");
     nb = prec == KSH_FP64 ? panel_width<double>(bw) : panel_width<float>(bw);
     if (nb == 0) return fail(KSH_SOLVER_ERROR, "band too wide for the one-workgroup factorization (LDS panel exceeds 160 KB)");
     return KSH_SUCCESS;
