@@ -10,10 +10,12 @@ TolF = 1e-8 (the metric's ||R||/||F|| bar), TolU = 1e-6, MaxIter 20, 60 steps ea
 stability (negative pivots every corrector iteration).
 
 A "step" is one converged arc-length step of one instance.  `value` = total converged steps of
-the 256 instances / wall seconds of the batched mixed form (FP64 K, FP32 band LDL^T with FP64
-refinement, all instances' device calls merged into batched launches).  The plain FP64 GPU form
-runs one instance at a time; it is timed on one instance per mesh size and scaled by the size
-histogram of the batch.  cpu_baseline: the FP64 CPU oracle port of the same loop (reference
+the 256 instances / wall seconds of the headline form (--form): "batched" = FP64 throughout with
+all instances' device calls merged into batched launches (default), "mixed" = the same with FP32
+band LDL^T and FP64 refinement.  Both batched forms are timed; the other one is reported beside
+the headline with its per-instance pivot counts and load factors compared against the FP64
+batch.  The plain FP64 GPU form runs one instance at a time; it is timed on one instance per mesh
+size and scaled by the size histogram of the batch.  cpu_baseline: the FP64 CPU oracle port of the same loop (reference
 re-factorisations kept) on a bounded prefix of one instance's path, on this host's cores.
 
 --workload c1: the single 32x32 perfect panel (the roof benchmark refined), plain vs mixed, 80 steps.
@@ -42,59 +44,37 @@ def log(*a):
 
 
 def kernel_stats(lib):
+    """(launches, total ms, algorithmic work) per profiled kernel, from the library's HIP-event
+    brackets on the launching stream and the work each launcher counts for its matrices."""
     import ctypes as C
     out = {}
     for nm in KERNELS:
         cnt = C.c_int64()
         ms = C.c_double()
+        work = C.c_double()
         lib.ksh_profile_query(nm.encode(), C.byref(cnt), C.byref(ms))
-        out[nm] = (cnt.value, ms.value)
+        lib.ksh_profile_query_work(nm.encode(), C.byref(work))
+        out[nm] = (cnt.value, ms.value, work.value)
     return out
 
 
-def work_per_launch(name, sps, esz):
-    """Algorithmic flops (mfma) or bytes (hbm) of one launch of `name` over the problems `sps`."""
-    tot = 0.0
-    for sp in sps:
-        n, bw, pt = sp.nfree, sp.bw, sp.patch
-        nb = ng = (pt.p + 1) ** 2
-        nel = pt.nx * pt.ny
-        w = {
-            "band_ldlt": n * (bw * bw + 3.0 * bw),                                   # n (b^2 + 3b) flop
-            "tangent_elem": nel * ng * 2.0 * (15 * 3 * nb * 5 + 9 * nb * nb * 5),     # Z and K contractions
-            "band_solve": 2.0 * n * (bw + 1) * esz + 3.0 * n * 8,                    # band read twice + x
-            "band_gather": nel * 9.0 * nb * nb * 8 + n * (bw + 1) * 8.0,             # element tiles + band (FP64)
-            "residual_elem": nel * (3.0 * nb * 8 + ng * 16 * 8),
-            "refine_residual_batch": n * (bw + 1) * 8.0 * 2 + 3.0 * n * 8,
-        }
-        tot += w.get(name, 0.0)
-    return tot
+# bound of each kernel: MFMA (work in flop) or HBM (work in bytes)
+BOUND = {"band_ldlt": "mfma", "band_ldlt_batch": "mfma", "tangent_elem": "mfma", "band_solve": "hbm",
+         "band_solve_batch": "hbm", "band_gather": "hbm", "band_symv": "hbm", "refine_residual_batch": "hbm"}
 
 
-BOUND = {"band_ldlt": "mfma", "tangent_elem": "mfma", "band_solve": "hbm", "band_gather": "hbm",
-         "residual_elem": "hbm", "refine_residual_batch": "hbm"}
-
-
-def roofline(stats, sps, res, fprec_fp32):
-    """Dominant kernel of the timed run against its bound.
-
-    achieved = algorithmic work per launch / average launch duration, where the work of all the
-    launches is counted from the solvers' own call counters (each tangent is assembled once and
-    factored once; each solve reads its band twice) and divided by the launch count."""
-    base = {"band_ldlt_batch": "band_ldlt", "band_solve_batch": "band_solve"}
-    cands = [k for k in stats if stats[k][0] > 0 and base.get(k, k) in BOUND]
-    best = max(cands, key=lambda k: stats[k][1])
-    kind = base.get(best, best)
-    cnt, ms = stats[best]
-    esz = 4 if fprec_fp32 else 8
-    calls = {"band_ldlt": "jacobians", "tangent_elem": "jacobians", "band_gather": "jacobians",
-             "residual_elem": "residuals", "band_solve": "solves", "refine_residual_batch": "solves"}[kind]
-    total_work = sum(r[calls] * work_per_launch(kind, [sp], esz) for sp, r in zip(sps, res))
-    per_launch = total_work / cnt
+def roofline(stats, fprec_fp32, kernel=None):
+    """The dominant kernel of the timed run (largest total time) against its roofline:
+    achieved = algorithmic work per launch (counted by the launcher over the matrices of each
+    launch) / average launch duration (HIP events on the launch stream)."""
+    cands = [k for k in stats if stats[k][0] > 0 and k in BOUND]
+    best = kernel or max(cands, key=lambda k: stats[k][1])
+    cnt, ms, work = stats[best]
+    per_launch = work / cnt
     avg_s = ms / cnt / 1e3
     achieved = per_launch / avg_s
-    if BOUND[kind] == "mfma":
-        peak = PEAK["fp32"] if (fprec_fp32 and kind == "band_ldlt") else PEAK["fp64"]
+    if BOUND[best] == "mfma":
+        peak = PEAK["fp32"] if (fprec_fp32 and best.startswith("band_ldlt")) else PEAK["fp64"]
         return {"kernel": best, "bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": None, "avg_launch_ms": ms / cnt, "launches": cnt,
                 "work_per_launch": per_launch, "work_unit": "flop"}
@@ -127,6 +107,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 reference timing")
+    ap.add_argument("--form", choices=["batched", "mixed"], default="batched", help="headline form")
+    ap.add_argument("--only", action="store_true", help="time the headline form only")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -141,7 +123,8 @@ def main():
     lib = L.load()
     kw = dict(method=0, max_iter=20, tol=1e-6, tol_f=1e-8, tol_u=1e-6, scaling=0.0, bifurcation_method=0,
               max_halvings=10)
-    mixed_kw = dict(precision=L.FP32_IR, max_refine=6, refine_tol=1e-11)
+    forms = {"batched": dict(precision=L.FP64), "mixed": dict(precision=L.FP32_IR, max_refine=6, refine_tol=1e-11)}
+    head_kw = forms[args.form]
 
     if args.workload == "c1":
         steps = args.steps or 80
@@ -164,27 +147,55 @@ def main():
 
     # warm-up: code objects, allocations (short separate run)
     if args.warmup > 0:
-        run_alm_batch(sps[: min(8, len(sps))], args.warmup, ds, **kw, **mixed_kw)
+        for f in forms.values():
+            run_alm_batch(sps[: min(8, len(sps))], args.warmup, ds, **kw, **f)
         sps[0].run_alm(args.warmup, ds, precision=L.FP64, **kw)
 
-    # timed: batched mixed form, per-kernel HIP events on the launching streams
+    def run_form(f):
+        t0 = time.perf_counter()
+        if len(sps) == 1:
+            r = [sps[0].run_alm(steps, ds, **kw, **forms[f])]
+        else:
+            r, _ = run_alm_batch(sps, steps, ds, **kw, **forms[f])
+        return r, time.perf_counter() - t0
+
+    # timed: headline form, per-kernel HIP events on the launching streams
     lib.ksh_profile_reset()
     lib.ksh_profile_enable(1)
-    t0 = time.perf_counter()
-    if len(sps) == 1:
-        res = [sps[0].run_alm(steps, ds, **kw, **mixed_kw)]
-    else:
-        res, _ = run_alm_batch(sps, steps, ds, **kw, **mixed_kw)
-    t_mixed = time.perf_counter() - t0
+    res, t_mixed = run_form(args.form)
     lib.ksh_profile_enable(0)
     stats = kernel_stats(lib)
     total_steps = int(sum(r["steps_done"] for r in res))
     value = total_steps / t_mixed
-    log(f"[bench] mixed{'+batched' if len(sps) > 1 else ''}: {total_steps} steps in {t_mixed:.3f} s "
+    log(f"[bench] {args.form}: {total_steps} steps in {t_mixed:.3f} s "
         f"-> {value:.2f} steps/s; corrector iterations {int(sum(r['iterations'].sum() for r in res))}")
-    for k, (c, ms) in stats.items():
+    kernels = {}
+    for k, (c, ms, work) in stats.items():
         if c:
             log(f"[bench]   {k:22s} launches {c:6d}  total {ms:10.2f} ms  avg {ms / c:9.4f} ms")
+            if k in BOUND and work > 0:
+                r = roofline(stats, args.form == "mixed", kernel=k)
+                kernels[k] = {"launches": c, "total_ms": round(ms, 2), "avg_ms": round(ms / c, 4),
+                              "achieved": round(r["achieved"], 4), "unit": r["unit"], "frac": round(r["frac"], 5)}
+
+    # the other batched form, compared instance by instance with the FP64 one
+    other = None
+    if not args.only:
+        of = "mixed" if args.form == "batched" else "batched"
+        res_o, t_o = run_form(of)
+        r64, rmx = (res, res_o) if args.form == "batched" else (res_o, res)
+        neg_same = sum(int(a["steps_done"] == b["steps_done"] and np.array_equal(a["negatives"], b["negatives"]))
+                       for a, b in zip(r64, rmx))
+        dlam = max(float(np.abs(a["lambda"][:a["steps_done"]] - b["lambda"][:a["steps_done"]]).max()
+                         / max(1e-300, np.abs(a["lambda"][:a["steps_done"]]).max()))
+                   for a, b in zip(r64, rmx) if a["steps_done"] == b["steps_done"] and a["steps_done"] > 0)
+        st_o = int(sum(r["steps_done"] for r in res_o))
+        other = {"form": of, "steps_per_s": round(st_o / t_o, 3), "converged_steps": st_o,
+                 "corrector_iterations": int(sum(r["iterations"].sum() for r in res_o)),
+                 "instances_with_identical_negatives_vs_fp64_batch": neg_same, "instances": len(sps),
+                 "max_rel_lambda_diff_vs_fp64_batch": dlam}
+        log(f"[bench] {of}: {st_o} steps in {t_o:.3f} s -> {st_o / t_o:.2f} steps/s; "
+            f"identical negatives {neg_same}/{len(sps)}, max rel lambda diff {dlam:.2e}")
 
     # plain FP64 GPU form: one instance at a time
     plain = None
@@ -240,17 +251,20 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (K, R, refinement, constraint) + f32 (band LDL^T)",
+        "dtype": "f64" if args.form == "batched" else "f64 (K, R, refinement, constraint) + f32 (band LDL^T)",
         "data": "synthetic",
         "config": {"workload": workload,
-                   "form": "mixed + batched" if len(sps) > 1 else "mixed, single instance",
+                   "form": {"batched": "FP64 + batched", "mixed": "mixed + batched"}[args.form]
+                           + ("" if len(sps) > 1 else ", single instance"),
                    "instances": len(sps), "parallelism": "single GPU"},
         "plain_fp64_steps_per_s": round(plain, 3) if plain else None,
         "speedup_vs_plain_fp64": round(value / plain, 3) if plain else None,
         "negatives_identical_sampled": same_neg,
         "converged_steps": total_steps,
         "corrector_iterations": int(sum(r["iterations"].sum() for r in res)),
-        "roofline": roofline(stats, sps, res, True),
+        "other_form": other,
+        "roofline": roofline(stats, args.form == "mixed"),
+        "kernels": kernels,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

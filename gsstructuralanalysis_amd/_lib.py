@@ -39,7 +39,8 @@ class AlmOptions(C.Structure):
                 ("scaling", C.c_double), ("angle_method", C.c_int), ("bifurcation_method", C.c_int),
                 ("relaxation", C.c_double), ("quasi", C.c_int), ("quasi_iterations", C.c_int),
                 ("adaptive", C.c_int), ("adaptive_iterations", C.c_int), ("max_refine", C.c_int),
-                ("refine_tol", C.c_double), ("max_halvings", C.c_int), ("verbose", C.c_int)]
+                ("refine_tol", C.c_double), ("max_halvings", C.c_int), ("verbose", C.c_int),
+                ("inertia_guard", C.c_double)]
 
 
 class AlmResult(C.Structure):
@@ -52,7 +53,8 @@ class AlmResult(C.Structure):
 
 class StaticOptions(C.Structure):
     _fields_ = [("precision", C.c_int), ("nsteps", C.c_int), ("dlambda", C.c_double), ("max_iter", C.c_int),
-                ("tol", C.c_double), ("max_refine", C.c_int), ("refine_tol", C.c_double), ("verbose", C.c_int)]
+                ("tol", C.c_double), ("max_refine", C.c_int), ("refine_tol", C.c_double), ("verbose", C.c_int),
+                ("inertia_guard", C.c_double)]
 
 
 class StaticResult(C.Structure):
@@ -96,6 +98,7 @@ SIGNATURES = {
     "ksh_profile_enable": (C.c_int, [C.c_int]),
     "ksh_profile_reset": (C.c_int, []),
     "ksh_profile_query": (C.c_int, [C.c_char_p, _i64p, _dp]),
+    "ksh_profile_query_work": (C.c_int, [C.c_char_p, _dp]),
     "ksh_run_alm": (C.c_int, [_VP, C.POINTER(AlmOptions), C.POINTER(AlmResult)]),
     "ksh_batch_run_alm": (C.c_int, [C.POINTER(_VP), C.c_int, C.POINTER(AlmOptions), C.POINTER(AlmResult), _dp]),
     "ksh_run_static": (C.c_int, [_VP, C.POINTER(StaticOptions), C.POINTER(StaticResult)]),

@@ -430,6 +430,7 @@ int ksh_matrix_create(ksh_problem *pb, int prec, ksh_matrix **out)
     K->prec = prec;
     K->sprec = storage_prec(prec);
     K->fprec = factor_prec(prec);
+    K->guard = pb->inertia_guard;
     int rc = 0;
     size_t count = (size_t)K->n * K->ld, eb = elem_bytes(K->sprec), fb = elem_bytes(K->fprec);
     auto bail = [&](int code) {

@@ -413,6 +413,12 @@ int run_tangent(hipStream_t s, InstDesc *d_desc, std::vector<InstDesc> &h, bool 
         else hipLaunchKernelGGL((k_mass_elem<3, float>), g, b, 0, s, d_desc, n);
     } else {
         ProfScope ps("tangent_elem", s);
+        double work = 0.0;  // B^T D B count per element and Gauss point: 72 n_e + 12 n_e^2
+        for (const auto &d : h) {
+            const double ng = (d.p + 1.0) * (d.p + 1.0), ne = 3.0 * ng;
+            work += (double)d.nel * ng * (72.0 * ne + 12.0 * ne * ne);
+        }
+        ps.set_work(work);
         if (p2 && f64) hipLaunchKernelGGL((k_tangent_elem<2, double>), g, b, 0, s, d_desc, n);
         else if (p2) hipLaunchKernelGGL((k_tangent_elem<2, float>), g, b, 0, s, d_desc, n);
         else if (f64) hipLaunchKernelGGL((k_tangent_elem<3, double>), g, b, 0, s, d_desc, n);
@@ -420,6 +426,12 @@ int run_tangent(hipStream_t s, InstDesc *d_desc, std::vector<InstDesc> &h, bool 
     }
     KSH_HIP(hipGetLastError());
     ProfScope ps(mass ? "mass_gather" : "band_gather", s);
+    {
+        const double es = f64 ? 8.0 : 4.0;  // element tiles read, band slots written
+        double work = 0.0;
+        for (const auto &d : h) work += (double)d.nel * kKeTile * es + (double)d.nfree * (d.bw + 1.0) * es;
+        ps.set_work(work);
+    }
     if (f64) hipLaunchKernelGGL(k_band_gather<double>, dim3(nblk(t.slots, 256)), b, 0, s, d_desc, n, t.slots);
     else hipLaunchKernelGGL(k_band_gather<float>, dim3(nblk(t.slots, 256)), b, 0, s, d_desc, n, t.slots);
     KSH_HIP(hipGetLastError());

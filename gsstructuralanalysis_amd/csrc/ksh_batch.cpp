@@ -358,6 +358,14 @@ private:
             ptot += 2 * (((size_t)g[i]->K->n + 63) / 64);
         }
         if (max_refine > 0 && (rc = m_part.ensure(ptot * sizeof(double)))) return rc;
+        // Refinement, instance by instance the same decisions as the single-problem solve
+        // (ksh_solve_dev): stop at ||r|| <= tol ||b||; on the last sweep of an FP32 factor whose
+        // residual is still above 1e-6 ||b||, replace the factor by an FP64 one and solve again.
+        std::vector<int> efp(g.size()), fallback(g.size(), 0), err(g.size(), 0);
+        for (size_t i = 0; i < g.size(); ++i) {
+            efp[i] = g[i]->K->use64 ? KSH_FP64 : g[i]->K->fprec;
+            if (g[i]->max_refine <= 0) active[i] = 0;
+        }
         for (int it = 0; it < max_refine; ++it) {
             std::vector<RefDesc> h(g.size());
             for (size_t i = 0; i < g.size(); ++i) {
@@ -369,17 +377,44 @@ private:
             std::vector<double> nr(2 * g.size());
             KSH_HIP(hipMemcpyAsync(nr.data(), m_norms.p, sizeof(double) * nr.size(), hipMemcpyDeviceToHost, m_stream));
             KSH_HIP(hipStreamSynchronize(m_stream));
+            std::vector<int> last(g.size(), 0);
             bool any = false;
             for (size_t i = 0; i < g.size(); ++i) {
                 if (!active[i]) continue;
-                double rn = std::sqrt(nr[2 * i]), bn = std::sqrt(nr[2 * i + 1]);
-                if (!(rn > g[i]->tol * bn) || it >= g[i]->max_refine) active[i] = 0;
-                else any = true;
+                const double rn = std::sqrt(nr[2 * i]), bn = std::sqrt(nr[2 * i + 1]);
+                const bool conv = rn <= g[i]->tol * bn;
+                if (!conv && !std::isfinite(rn)) {
+                    err[i] = 1;
+                    active[i] = 0;
+                } else if (conv) {
+                    active[i] = 0;
+                } else if (it == g[i]->max_refine - 1 && efp[i] == KSH_FP32 && rn > 1e-6 * bn) {
+                    fallback[i] = 1;
+                    active[i] = 0;
+                } else {
+                    last[i] = it == g[i]->max_refine - 1;
+                    any = true;
+                }
             }
             if (!any) break;
             if ((rc = solve_all(r, active))) return rc;
             for (size_t i = 0; i < g.size(); ++i) h[i].active = active[i];
             if ((rc = refine_update_batch(m_stream, (RefDesc *)m_rdesc.p, h))) return rc;
+            for (size_t i = 0; i < g.size(); ++i)
+                if (last[i]) active[i] = 0;
+        }
+        // FP64 fallback of stalled refinements: x = K64^-1 b
+        {
+            std::vector<BandDescHost> h64;
+            for (size_t i = 0; i < g.size(); ++i) {
+                if (!fallback[i]) continue;
+                Matrix &K = *g[i]->K;
+                KSH_HIP(hipStreamSynchronize(m_stream));  // the problem's stream reads the engine's K
+                if ((rc = refactor_fp64(K))) return rc;
+                KSH_HIP(hipMemcpyAsync(x + off[i], b + off[i], sizeof(double) * K.n, hipMemcpyDeviceToDevice, m_stream));
+                h64.push_back(BandDescHost{K.fac64, K.n, K.bw, nullptr, nullptr, x + off[i], 1});
+            }
+            if (!h64.empty() && (rc = band_solve_batch(m_stream, m_bdesc.p, h64, KSH_FP64))) return rc;
         }
         KSH_HIP(hipMemcpyAsync(m_hout.p, x, m_tot * sizeof(double), hipMemcpyDeviceToHost, m_stream));
         KSH_HIP(hipStreamSynchronize(m_stream));
@@ -387,7 +422,7 @@ private:
         for (size_t i = 0; i < g.size(); ++i) {
             int n = g[i]->pb->nfree;
             std::memcpy(g[i]->out, ho + off[i], sizeof(double) * n);
-            g[i]->rc = KSH_SUCCESS;
+            g[i]->rc = err[i] ? fail(KSH_SOLVER_ERROR, "refinement diverged") : KSH_SUCCESS;
             for (int k = 0; k < n; ++k)
                 if (!std::isfinite(ho[off[i] + k])) {
                     g[i]->rc = fail(KSH_SOLVER_ERROR, "non-finite solution");

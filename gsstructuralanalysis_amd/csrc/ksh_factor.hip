@@ -402,6 +402,7 @@ int band_factor(Matrix &K, void *fac, int prec, double *stats_out)
     double *dstats = K.red + 4;
     {
         ProfScope ps("band_ldlt", pb.stream);
+        ps.set_work((double)K.n * ((double)K.bw * K.bw + 3.0 * K.bw));
         rc = prec == KSH_FP64 ? launch_factor<double>(pb.stream, nb, 1, nullptr, (double *)fac, K.n, K.bw, K.flag, dstats)
                               : launch_factor<float>(pb.stream, nb, 1, nullptr, (float *)fac, K.n, K.bw, K.flag, dstats);
     }
@@ -435,6 +436,12 @@ int band_factor_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHo
     nb = prec == KSH_FP64 ? panel_width<double>(maxbw) : panel_width<float>(maxbw);
     KSH_HIP(hipMemcpyAsync(d_descs, dd.data(), sizeof(BandDesc) * dd.size(), hipMemcpyHostToDevice, s));
     ProfScope ps("band_ldlt_batch", s);
+    {
+        double work = 0.0;  // n (b^2 + 3b) flop per matrix
+        for (const auto &x : h)
+            if (x.active) work += (double)x.n * ((double)x.bw * x.bw + 3.0 * x.bw);
+        ps.set_work(work);
+    }
     return prec == KSH_FP64 ? launch_factor<double>(s, nb, (unsigned)dd.size(), (const BandDesc *)d_descs, nullptr, 0, maxbw,
                                                     nullptr, nullptr)
                             : launch_factor<float>(s, nb, (unsigned)dd.size(), (const BandDesc *)d_descs, nullptr, 0, maxbw,

@@ -74,6 +74,7 @@ struct Problem {
     double fsurf[3] = {0, 0, 0};
     int device = 0;
     hipStream_t stream = nullptr;
+    double inertia_guard = 1e-6;   // guard given to the matrices created for this problem
 
     double *ctrl0 = nullptr;
     int *free_idx = nullptr;

@@ -20,6 +20,7 @@ struct Entry {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
     double total_ms = 0.0;
     long long count = 0;
+    double work = 0.0;
 };
 
 std::mutex g_mu;
@@ -73,7 +74,9 @@ ProfScope::~ProfScope()
     if (!m_active) return;
     std::lock_guard<std::mutex> lk(g_mu);
     (void)hipEventRecord(m_stop, m_stream);
-    g_entries[m_name].pending.emplace_back(m_start, m_stop);
+    Entry &en = g_entries[m_name];
+    en.pending.emplace_back(m_start, m_stop);
+    en.work += m_work;
 }
 
 }  // namespace ksh
@@ -108,6 +111,15 @@ int ksh_profile_query(const char *kernel, int64_t *launches, double *total_ms)
     ksh::drain(it->second);
     if (launches) *launches = it->second.count;
     if (total_ms) *total_ms = it->second.total_ms;
+    return KSH_SUCCESS;
+}
+
+int ksh_profile_query_work(const char *kernel, double *work)
+{
+    if (!kernel || !work) return ksh::fail(KSH_OTHER_ERROR, "null argument");
+    std::lock_guard<std::mutex> lk(ksh::g_mu);
+    auto it = ksh::g_entries.find(kernel);
+    *work = it == ksh::g_entries.end() ? 0.0 : it->second.work;
     return KSH_SUCCESS;
 }
 

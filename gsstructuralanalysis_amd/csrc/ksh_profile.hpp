@@ -12,6 +12,8 @@ public:
     ProfScope(const char *name, hipStream_t s);
     ~ProfScope();
     ProfScope(const ProfScope &) = delete;
+    // algorithmic work of the bracketed launch (flops or bytes, as the kernel's bound counts it)
+    void set_work(double w) { m_work = w; }
     ProfScope &operator=(const ProfScope &) = delete;
 
 private:
@@ -19,6 +21,7 @@ private:
     hipStream_t m_stream;
     hipEvent_t m_start = nullptr, m_stop = nullptr;
     bool m_active = false;
+    double m_work = 0.0;
 };
 
 }  // namespace ksh

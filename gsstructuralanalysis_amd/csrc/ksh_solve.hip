@@ -391,6 +391,12 @@ int band_solve_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHos
     KSH_HIP(hipMemcpyAsync(d_descs, dd.data(), sizeof(BandDesc) * dd.size(), hipMemcpyHostToDevice, s));
     const size_t smem = solve_lds(nmax, bwmax, bytes_max, prec);
     ProfScope ps("band_solve_batch", s);
+    {
+        double work = 0.0;  // band read twice, vector in and out
+        for (const auto &x : h)
+            if (x.active) work += 2.0 * x.n * (x.bw + 1.0) * es + 16.0 * x.n;
+        ps.set_work(work);
+    }
     if (smem) {
         int rc = allow_solve_lds(smem);
         if (rc) return rc;
@@ -416,6 +422,7 @@ int band_solve(Matrix &K, const void *fac, int prec, double *x_dev)
     Problem &pb = *K.pb;
     const size_t smem = solve_lds(K.n, K.bw, (size_t)K.n * K.ld * (prec == KSH_FP64 ? sizeof(double) : sizeof(float)), prec);
     ProfScope ps("band_solve", pb.stream);
+    ps.set_work(2.0 * K.n * (K.bw + 1.0) * (prec == KSH_FP64 ? 8.0 : 4.0) + 16.0 * K.n);
     if (smem) {
         int rc = allow_solve_lds(smem);
         if (rc) return rc;

@@ -169,6 +169,7 @@ int band_symv(const Matrix &K, const double *x_dev, double *y_dev)
 {
     Problem &pb = *K.pb;
     ProfScope ps("band_symv", pb.stream);
+    ps.set_work(K.n * (2.0 * K.bw + 1.0) * (K.sprec == KSH_FP64 ? 8.0 : 4.0) + 16.0 * K.n);
     if (K.sprec == KSH_FP64)
         hipLaunchKernelGGL(k_symv_rows<double>, dim3(nblocks(K.n)), dim3(kT), 0, pb.stream, (const double *)K.ab, K.n, K.bw,
                            x_dev, y_dev);
@@ -187,6 +188,10 @@ int refine_residual_batch(hipStream_t s, RefDesc *d_descs, const std::vector<Ref
     KSH_HIP(hipMemcpyAsync(d_descs, h.data(), sizeof(RefDesc) * h.size(), hipMemcpyHostToDevice, s));
     {
         ProfScope ps("refine_residual_batch", s);
+        double work = 0.0;  // K read as a full band (2b+1 per row), x and b read, r written
+        for (const auto &d : h)
+            if (d.active) work += d.n * (2.0 * d.bw + 1.0) * (d.sprec == KSH_FP64 ? 8.0 : 4.0) + 24.0 * d.n;
+        ps.set_work(work);
         hipLaunchKernelGGL(k_refine_rows_batched, dim3((unsigned)tot), dim3(kT), 0, s, (const RefDesc *)d_descs, (int)h.size());
     }
     KSH_HIP(hipGetLastError());

@@ -28,11 +28,12 @@ def _f64(a):
 
 def alm_options(nsteps, length, method=0, precision=L.FP64, max_iter=20, tol=1e-6, tol_u=1e-6, tol_f=1e-3,
                 scaling=-1.0, angle_method=0, bifurcation_method=0, relaxation=1.0, quasi=0, quasi_iterations=-1,
-                adaptive=0, adaptive_iterations=10, max_refine=0, refine_tol=1e-14, max_halvings=10, verbose=0):
+                adaptive=0, adaptive_iterations=10, max_refine=0, refine_tol=1e-14, max_halvings=10, verbose=0,
+                inertia_guard=0.0):
     """ksh_alm_options with the reference's option names in snake case (gsALMBase.hpp:22-54)."""
     return L.AlmOptions(method, precision, nsteps, length, max_iter, tol, tol_u, tol_f, scaling, angle_method,
                         bifurcation_method, relaxation, quasi, quasi_iterations, adaptive, adaptive_iterations,
-                        max_refine, refine_tol, max_halvings, verbose)
+                        max_refine, refine_tol, max_halvings, verbose, inertia_guard)
 
 
 def _alm_result(nsteps, nfree, keep_U, probe_dof):
@@ -211,8 +212,8 @@ class ShellProblem:
         return _alm_collect(r, out, U, nsteps, self.nfree)
 
     def run_static(self, nsteps, dlambda, precision=L.FP64, max_iter=25, tol=1e-10, max_refine=0,
-                   refine_tol=1e-14, verbose=0, keep_U=False):
-        o = L.StaticOptions(precision, nsteps, dlambda, max_iter, tol, max_refine, refine_tol, verbose)
+                   refine_tol=1e-14, verbose=0, keep_U=False, inertia_guard=0.0):
+        o = L.StaticOptions(precision, nsteps, dlambda, max_iter, tol, max_refine, refine_tol, verbose, inertia_guard)
         out = {k: np.zeros(nsteps) for k in ("lambda", "unorm", "residual")}
         out["iterations"] = np.zeros(nsteps, dtype=np.int32)
         out["negatives"] = np.zeros(nsteps, dtype=np.int64)

@@ -140,11 +140,18 @@ int ksh_set_inertia_guard(ksh_matrix *K, double guard);
 /* ---- measurement --------------------------------------------------------------
  * No counterpart in the reference (it only accumulates wall time inside its lambdas,
  * benchmarks/benchmark_Roof.cpp:316-343).  When enabled, every launch of the named
- * kernels ("tangent_elem", "band_gather", "residual_elem", "vector_gather",
- * "band_ldlt", "band_solve", "band_symv") is bracketed by HIP events on its stream. */
+ * kernels ("tangent_elem", "band_gather", "residual_elem", "vector_gather", "band_ldlt",
+ * "band_ldlt_batch", "band_solve", "band_solve_batch", "band_symv", "refine_residual_batch")
+ * is bracketed by HIP events on its stream. */
 int ksh_profile_enable(int on);
 int ksh_profile_reset(void);
 int ksh_profile_query(const char *kernel, int64_t *launches, double *total_ms);
+/* Algorithmic work summed over the profiled launches of `kernel`: flops for the MFMA-bound
+ * kernels (band_ldlt*: n (b^2 + 3b) per matrix; tangent_elem: n_el n_g (72 n_e + 12 n_e^2)),
+ * bytes for the HBM-bound ones (band_solve*: 2 n (b+1) sizeof(factor) + 16 n per matrix;
+ * refine_residual_batch / band_symv: n (2b+1) sizeof(K) + 24 n; band_gather: element tiles
+ * read + band written). */
+int ksh_profile_query_work(const char *kernel, double *work);
 
 /* ---- solver drivers ---------------------------------------------------------
  * The reference's solver classes (gsALMCrisfield / gsALMRiks, gsStaticNewton +
@@ -167,6 +174,9 @@ typedef struct {
     double refine_tol;       /* stop refinement at ||b - K x|| <= refine_tol ||b||         */
     int max_halvings;        /* driver: halve the arc length on failure, retry            */
     int verbose;
+    double inertia_guard;    /* FP32 factors: a factor whose min |d| < guard max |d| is   */
+                             /* replaced by an FP64 one for its solves and pivot count    */
+                             /* (0: the library default, 1e-6)                            */
 } ksh_alm_options;
 
 typedef struct {
@@ -208,6 +218,7 @@ typedef struct {
     int max_refine;
     double refine_tol;
     int verbose;
+    double inertia_guard;    /* as in ksh_alm_options                                      */
 } ksh_static_options;
 
 typedef struct {
