@@ -109,7 +109,22 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 reference timing")
     ap.add_argument("--form", choices=["batched", "mixed"], default="batched", help="headline form")
     ap.add_argument("--only", action="store_true", help="time the headline form only")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-MiB device buffer 5 times (device-to-device), for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log(f"[bench] copied {args.copy_calibration} MiB x 5 (reads {args.copy_calibration * 1024} KiB and writes "
+            f"{args.copy_calibration * 1024} KiB per copy)")
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: measures one GPU only; instances of a batch would be sharded over GPUs "

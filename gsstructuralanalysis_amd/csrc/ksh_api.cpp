@@ -444,7 +444,10 @@ int ksh_matrix_create(ksh_problem *pb, int prec, ksh_matrix **out)
     if (!(K->flag = dalloc<int>(4, rc))) return bail(rc);
     if (!(K->red = dalloc<double>(8, rc))) return bail(rc);
     if (K->sprec == KSH_FP32 && !(K->u_at = dalloc<double>(K->n, rc))) return bail(rc);
-    if (hipMemsetAsync(K->ab, 0, count * eb, pb->stream) != hipSuccess) return bail(fail(KSH_OTHER_ERROR, "memset"));
+    // zeroed before return: the matrix may next be written on another stream (a batch engine's),
+    // which must not race with this clear
+    if (hipMemsetAsync(K->ab, 0, count * eb, pb->stream) != hipSuccess || hipStreamSynchronize(pb->stream) != hipSuccess)
+        return bail(fail(KSH_OTHER_ERROR, "memset"));
     *out = K;
     return KSH_SUCCESS;
 }
