@@ -46,6 +46,7 @@ struct Req {
     double *dmin = nullptr;
     int rc = 0;
     bool done = false;
+    int waited = 0;  // engine rounds this request has been left pending
 };
 
 struct DevBuf {
@@ -126,7 +127,7 @@ public:
                 std::unique_lock<std::mutex> lk(m_mu);
                 m_cv_main.wait(lk, [&] { return m_active == 0 || (m_blocked == m_active && !m_pending.empty()); });
                 if (m_active == 0 && m_pending.empty()) break;
-                batch.swap(m_pending);
+                select(batch);
             }
             int rc = execute(batch);
             if (rc && !first_error) first_error = rc;
@@ -146,6 +147,41 @@ public:
     long long rounds() const { return m_rounds; }
 
 private:
+    // Requests of one kind per round: the kind most instances are waiting for (inertia queries,
+    // answered from the factor's pivot statistics, always go along).  Instances that want another
+    // kind wait, so the instances fall into step and each launch carries most of the batch; a
+    // request left waiting kMaxWait rounds has its kind taken next.
+    void select(std::vector<Req *> &batch)
+    {
+        constexpr int kMaxWait = 16;
+        int cnt[6] = {0, 0, 0, 0, 0, 0};
+        int forced = -1, oldest = -1;
+        for (Req *r : m_pending) {
+            ++cnt[r->kind];
+            if (r->waited >= kMaxWait && r->waited > oldest) {
+                oldest = r->waited;
+                forced = r->kind;
+            }
+        }
+        int kind = forced;
+        if (kind < 0) {
+            kind = RQ_TANGENT;
+            for (int k = 0; k < 6; ++k)
+                if (k != RQ_INERTIA && cnt[k] > cnt[kind]) kind = k;
+            if (cnt[kind] == 0) kind = RQ_INERTIA;
+        }
+        std::vector<Req *> keep;
+        for (Req *r : m_pending) {
+            if (r->kind == kind || r->kind == RQ_INERTIA) {
+                batch.push_back(r);
+            } else {
+                ++r->waited;
+                keep.push_back(r);
+            }
+        }
+        m_pending.swap(keep);
+    }
+
     int execute(std::vector<Req *> &batch)
     {
         std::vector<Req *> by[6];
