@@ -63,6 +63,18 @@ BOUND = {"band_ldlt": "mfma", "band_ldlt_batch": "mfma", "tangent_elem": "mfma",
          "band_solve_batch": "hbm", "band_gather": "hbm", "band_symv": "hbm", "refine_residual_batch": "hbm"}
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    (scaled by a device copy of known size), or None when no such measurement is recorded."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)["kernels"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+    return None if rec is None else float(rec["traffic"])
+
+
 def roofline(stats, fprec_fp32, kernel=None):
     """The dominant kernel of the timed run (largest total time) against its roofline:
     achieved = algorithmic work per launch (counted by the launcher over the matrices of each
@@ -76,10 +88,10 @@ def roofline(stats, fprec_fp32, kernel=None):
     if BOUND[best] == "mfma":
         peak = PEAK["fp32"] if (fprec_fp32 and best.startswith("band_ldlt")) else PEAK["fp64"]
         return {"kernel": best, "bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None, "avg_launch_ms": ms / cnt, "launches": cnt,
+                "frac": achieved / peak, "traffic": measured_traffic(best), "avg_launch_ms": ms / cnt, "launches": cnt,
                 "work_per_launch": per_launch, "work_unit": "flop"}
     return {"kernel": best, "bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK["hbm"] / 1e9, "unit": "GB/s",
-            "frac": achieved / PEAK["hbm"], "traffic": None, "avg_launch_ms": ms / cnt, "launches": cnt,
+            "frac": achieved / PEAK["hbm"], "traffic": measured_traffic(best), "avg_launch_ms": ms / cnt, "launches": cnt,
             "work_per_launch": per_launch, "work_unit": "byte"}
 
 

@@ -15,6 +15,7 @@ FP64, FP32_IR, FP32 = 0, 1, 2
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 _i64p = C.POINTER(C.c_int64)
+_fp = C.POINTER(C.c_float)
 
 
 class Patch(C.Structure):
@@ -63,6 +64,20 @@ class StaticResult(C.Structure):
                 ("wall_seconds", C.c_double), ("jacobians", C.c_int64)]
 
 
+EIG_SHIFT_INVERT, EIG_BUCKLING = 0, 1
+
+
+class EigOptions(C.Structure):
+    _fields_ = [("mode", C.c_int), ("number", C.c_int), ("ncv_fac", C.c_int), ("shift", C.c_double),
+                ("tol", C.c_double), ("max_restarts", C.c_int), ("precision", C.c_int), ("max_refine", C.c_int),
+                ("refine_tol", C.c_double), ("seed", C.c_uint64)]
+
+
+class EigResult(C.Structure):
+    _fields_ = [("converged", C.c_int), ("restarts", C.c_int), ("solves", C.c_int64), ("products", C.c_int64),
+                ("wall_seconds", C.c_double)]
+
+
 # every symbol declared in include/ksh.h: name -> (restype, argtypes)
 _VP = C.c_void_p
 SIGNATURES = {
@@ -102,6 +117,17 @@ SIGNATURES = {
     "ksh_run_alm": (C.c_int, [_VP, C.POINTER(AlmOptions), C.POINTER(AlmResult)]),
     "ksh_batch_run_alm": (C.c_int, [C.POINTER(_VP), C.c_int, C.POINTER(AlmOptions), C.POINTER(AlmResult), _dp]),
     "ksh_run_static": (C.c_int, [_VP, C.POINTER(StaticOptions), C.POINTER(StaticResult)]),
+    "ksh_eigen": (C.c_int, [_VP, _VP, C.POINTER(EigOptions), _dp, _dp, C.POINTER(EigResult)]),
+    "ksh_run_buckling": (C.c_int, [_VP, C.POINTER(EigOptions), _dp, _dp, C.POINTER(EigResult)]),
+    "ksh_run_modal": (C.c_int, [_VP, C.POINTER(EigOptions), _dp, _dp, C.POINTER(EigResult)]),
+    "ksh_mlp_create": (C.c_int, [_ip, C.c_uint64, C.POINTER(_VP)]),
+    "ksh_mlp_destroy": (C.c_int, [_VP]),
+    "ksh_mlp_num_params": (C.c_int, [_VP, _i64p]),
+    "ksh_mlp_get_params": (C.c_int, [_VP, _fp]),
+    "ksh_mlp_set_params": (C.c_int, [_VP, _fp]),
+    "ksh_mlp_forward": (C.c_int, [_VP, _fp, C.c_int, _fp]),
+    "ksh_mlp_gradients": (C.c_int, [_VP, _fp, _fp, C.c_int, _fp, _dp]),
+    "ksh_mlp_train": (C.c_int, [_VP, _fp, _fp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_uint64, _dp]),
 }
 
 _lib = None

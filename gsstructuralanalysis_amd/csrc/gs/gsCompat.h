@@ -205,6 +205,33 @@ public:
         return y;
     }
 
+    // deep copy into a new device matrix of the same problem and precision
+    gsSparseMatrix clone() const
+    {
+        auto s = std::make_shared<gsMatrixStore>();
+        s->pb = m_s->pb;
+        s->prec = m_s->prec;
+        s->exec = m_s->exec;
+        if (ksh_matrix_create(s->pb, s->prec, &s->K) != KSH_SUCCESS || ksh_matrix_copy(s->K, m_s->K) != KSH_SUCCESS)
+            throw std::runtime_error(ksh_last_error());
+        return gsSparseMatrix(s);
+    }
+    // this = alpha this + beta o, in place (A - sigma B and B -= A of
+    // src/gsEigenSolvers/gsEigenProblemBase.hpp:32 and gsBucklingSolver.h:104,144)
+    gsSparseMatrix &axpby(T alpha, T beta, const gsSparseMatrix &o)
+    {
+        if (ksh_matrix_axpby(handle(), alpha, beta, o.handle()) != KSH_SUCCESS) throw std::runtime_error(ksh_last_error());
+        return *this;
+    }
+    gsSparseMatrix &operator-=(const gsSparseMatrix &o) { return axpby(T(1), T(-1), o); }
+    gsSparseMatrix &operator+=(const gsSparseMatrix &o) { return axpby(T(1), T(1), o); }
+    friend gsSparseMatrix operator-(const gsSparseMatrix &a, const gsSparseMatrix &b)
+    {
+        gsSparseMatrix r = a.clone();
+        r -= b;
+        return r;
+    }
+
 private:
     std::shared_ptr<gsMatrixStore> m_s;
 };

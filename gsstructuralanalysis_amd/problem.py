@@ -86,6 +86,29 @@ def run_alm_batch(problems, nsteps, length, keep_U=False, **kw):
     return results, wall.value
 
 
+def eig_options(number=10, mode=L.EIG_SHIFT_INVERT, ncv_fac=3, shift=0.0, tol=1e-10, max_restarts=0,
+                precision=L.FP64, max_refine=8, refine_tol=1e-13, seed=0):
+    """ksh_eig_options with gsEigenProblemBase's option meanings (gsEigenProblemBase.h:40-89)."""
+    return L.EigOptions(mode, number, ncv_fac, shift, tol, max_restarts, precision,
+                        max_refine if precision != L.FP64 else 0, refine_tol, seed)
+
+
+def _eig_collect(values, vectors, res):
+    return {"values": values, "vectors": vectors, "converged": res.converged, "restarts": res.restarts,
+            "solves": res.solves, "products": res.products, "wall_seconds": res.wall_seconds}
+
+
+def eigen(A: "BandMatrix", B: "BandMatrix", number=10, keep_vectors=False, **kw):
+    """Eigenpairs of A x = lambda B x nearest `shift` (ksh_eigen)."""
+    o = eig_options(number, **kw)
+    vals = np.zeros(number)
+    vecs = np.zeros((number, A.n)) if keep_vectors else None
+    res = L.EigResult()
+    L.check(L.load().ksh_eigen(A.handle, B.handle, C.byref(o), _dptr(vals),
+                               _dptr(vecs) if vecs is not None else None, C.byref(res)))
+    return _eig_collect(vals, vecs, res)
+
+
 class BandMatrix:
     """Device band matrix (lower band, ab[J*(bw+1)+(I-J)]) with its LDL^T factor."""
 
@@ -210,6 +233,24 @@ class ShellProblem:
         if code not in (L.SUCCESS, L.NOT_CONVERGED):
             L.check(code)
         return _alm_collect(r, out, U, nsteps, self.nfree)
+
+    def _run_eig(self, fn, number, keep_vectors, kw):
+        o = eig_options(number, **kw)
+        vals = np.zeros(number)
+        vecs = np.zeros((number, self.nfree)) if keep_vectors else None
+        res = L.EigResult()
+        L.check(fn(self.handle, C.byref(o), _dptr(vals), _dptr(vecs) if vecs is not None else None, C.byref(res)))
+        return _eig_collect(vals, vecs, res)
+
+    def run_buckling(self, number=10, keep_vectors=False, **kw):
+        """solvers/buckling_shell_XML.cpp:212-247 on the device: K_L, F, u = K_L^-1 F, K_NL = K(u),
+        gsBucklingSolver(K_L, K_NL).computeSparse(number)."""
+        kw.setdefault("mode", L.EIG_BUCKLING)
+        return self._run_eig(L.load().ksh_run_buckling, number, keep_vectors, kw)
+
+    def run_modal(self, number=10, keep_vectors=False, **kw):
+        """solvers/modal_shell_XML.cpp:233-253 on the device: gsModalSolver(K, M).computeSparse(number)."""
+        return self._run_eig(L.load().ksh_run_modal, number, keep_vectors, kw)
 
     def run_static(self, nsteps, dlambda, precision=L.FP64, max_iter=25, tol=1e-10, max_refine=0,
                    refine_tol=1e-14, verbose=0, keep_U=False, inertia_guard=0.0):

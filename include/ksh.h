@@ -238,6 +238,70 @@ typedef struct {
  * K_L, F from assemble(); gsStaticNewton(K_L, F, Jacobian, ALResidual); gsControlDisplacement. */
 int ksh_run_static(ksh_problem *pb, const ksh_static_options *opt, ksh_static_result *res);
 
+/* ---- eigen-solvers --------------------------------------------------------------
+ * Generalized symmetric eigenpairs A x = lambda B x nearest `shift`, by thick-restart Lanczos
+ * on a shift-invert operator whose solves use the GPU band factor of A - shift B.  Replaces
+ * gsEigenProblemBase::computeSparse (src/gsEigenSolvers/gsEigenProblemBase.hpp:152-216,
+ * Spectra SymGEigsShiftSolver) under gsModalSolver (A = K, B = M; mode SHIFT_INVERT, B SPD)
+ * and gsBucklingSolver (A = K_L, B = K_NL - K_L; mode BUCKLING, A SPD). */
+enum ksh_eig_mode {
+    KSH_EIG_SHIFT_INVERT = 0, /* S = (A - s B)^-1 B, B-inner product ("solver" 0, 1, 2)   */
+    KSH_EIG_BUCKLING = 1      /* S = A^-1 B (s = 0) or (A - s B)^-1 A, A-inner product (3) */
+};
+
+typedef struct {
+    int mode;                /* ksh_eig_mode                                              */
+    int number;              /* eigenpairs wanted ("number" of computeSparse)             */
+    int ncv_fac;             /* Lanczos basis size = ncv_fac * number ("ncvFac")         */
+    double shift;            /* "shift"                                                   */
+    double tol;              /* "tolerance": Ritz residual <= tol |theta|                 */
+    int max_restarts;        /* 0: 1000 (Spectra's maxit of computeSparse_impl)           */
+    int precision;           /* factor of A - shift B: KSH_FP64 | KSH_FP32_IR            */
+    int max_refine;          /* refinement sweeps per solve (FP32_IR)                     */
+    double refine_tol;
+    uint64_t seed;           /* start vector (0: fixed default)                          */
+} ksh_eig_options;
+
+typedef struct {
+    int converged;           /* pairs meeting tol                                         */
+    int restarts;
+    int64_t solves, products;/* operator solves and W mat-vecs                            */
+    double wall_seconds;
+} ksh_eig_result;
+
+/* values[number] ascending; vectors[number*n] (optional, W-normalized, free dofs) */
+int ksh_eigen(ksh_matrix *A, ksh_matrix *B, const ksh_eig_options *opt, double *values, double *vectors,
+              ksh_eig_result *res);
+
+/* solvers/buckling_shell_XML.cpp:212-247: K_L, F = assemble(); u = K_L^-1 F; K_NL = K(u);
+ * gsBucklingSolver(K_L, K_NL).computeSparse(number) (opt->mode is taken as BUCKLING). */
+int ksh_run_buckling(ksh_problem *pb, const ksh_eig_options *opt, double *values, double *vectors,
+                     ksh_eig_result *res);
+
+/* solvers/modal_shell_XML.cpp:233-253: K = assemble().matrix(), M = assembleMass();
+ * gsModalSolver(K, M).computeSparse(number) (opt->mode is taken as SHIFT_INVERT). */
+int ksh_run_modal(ksh_problem *pb, const ksh_eig_options *opt, double *values, double *vectors,
+                  ksh_eig_result *res);
+
+/* ---- learned predictor network ------------------------------------------------------
+ * The MLP of the learned arc-length predictor (no reference counterpart; it enters the solver
+ * through the protected predictor() of gsALMCrisfield, src/gsALMSolvers/gsALMCrisfield.h:85,100).
+ * widths[4] = {264, 256, 256, 65}: GELU hidden layers, linear output; BF16 products with FP32
+ * accumulation on MFMA, FP32 master weights, Adam.  Parameters are packed as W0, W1, W2
+ * (row-major [in][out]) then b0, b1, b2.  Host arrays, row-major samples. */
+typedef struct ksh_mlp ksh_mlp;
+int ksh_mlp_create(const int *widths, uint64_t seed, ksh_mlp **out);
+int ksh_mlp_destroy(ksh_mlp *m);
+int ksh_mlp_num_params(const ksh_mlp *m, int64_t *n);
+int ksh_mlp_get_params(const ksh_mlp *m, float *out);
+int ksh_mlp_set_params(ksh_mlp *m, const float *in);
+int ksh_mlp_forward(ksh_mlp *m, const float *x, int batch, float *y);
+/* gradient of the mean squared error (over batch x outputs) with respect to the parameters */
+int ksh_mlp_gradients(ksh_mlp *m, const float *x, const float *target, int batch, float *grads, double *loss);
+/* minibatch Adam over the samples (full batches per epoch, shuffled with `seed`) */
+int ksh_mlp_train(ksh_mlp *m, const float *x, const float *target, int nsamples, int epochs, int batch, double lr,
+                  double weight_decay, uint64_t seed, double *loss_per_epoch);
+
 #ifdef __cplusplus
 }
 #endif

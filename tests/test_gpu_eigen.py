@@ -1,0 +1,81 @@
+"""GPU eigen-solvers (gsModalSolver / gsBucklingSolver on the band factor) against dense FP64
+generalized eigenvalues of the oracle's matrices (scipy.linalg.eigh) and the closed-form
+simply supported plate frequencies (src/gsEigenSolvers/gsModalSolver.h, gsBucklingSolver.h;
+solvers/modal_shell_XML.cpp, solvers/buckling_shell_XML.cpp).  Bar: eigenvalues within 1e-8
+relative of the FP64 oracle, in the plain and in the FP32-factor form."""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+from gsstructuralanalysis_amd import _lib as L
+from gsstructuralanalysis_amd import geometry as g
+from oracle.ok import Oracle, band_to_dense
+
+pytestmark = pytest.mark.gpu
+
+
+def _sp(pt):
+    from gsstructuralanalysis_amd.problem import ShellProblem
+    return ShellProblem(pt)
+
+
+@pytest.mark.parametrize("precision", [L.FP64, L.FP32_IR])
+def test_modal_matches_dense_oracle(precision, ksh):
+    pt = g.simply_supported_plate_problem(p=3, n=8, rho=1e3)
+    o = Oracle(pt)
+    print(f"[case] modal SS plate p=3 n=8 nfree={o.nfree} precision={precision}", flush=True)
+    K = band_to_dense(o.tangent_band(np.zeros(o.nfree)), o.nfree, o.bw)
+    M = band_to_dense(o.mass_band(), o.nfree, o.bw)
+    ref = sl.eigh(K, M, eigvals_only=True)[:10]
+    r = _sp(pt).run_modal(10, precision=precision, tol=1e-12)
+    assert r["converged"] == 10
+    assert np.abs(r["values"] - ref).max() <= 1e-8 * np.abs(ref).max()
+
+
+def test_modal_eigenvectors_and_shift(ksh):
+    """Pairs nearest a shift inside the spectrum; eigenvector residuals ||K x - lambda M x||."""
+    from gsstructuralanalysis_amd.problem import BandMatrix, eigen
+    pt = g.simply_supported_plate_problem(p=3, n=8, rho=1e3)
+    o = Oracle(pt)
+    sp = _sp(pt)
+    K = sp.tangent(np.zeros(o.nfree))
+    M = sp.mass()
+    Kd = band_to_dense(K.download(), o.nfree, o.bw)
+    Md = band_to_dense(M.download(), o.nfree, o.bw)
+    ev = sl.eigh(Kd, Md, eigvals_only=True)
+    shift = 0.5 * (ev[6] + ev[7])
+    r = eigen(K, M, number=6, shift=shift, tol=1e-12, keep_vectors=True)
+    ref = np.sort(ev[np.argsort(np.abs(ev - shift))[:6]])
+    assert np.abs(r["values"] - ref).max() <= 1e-8 * np.abs(ref).max()
+    for lam, x in zip(r["values"], r["vectors"]):
+        res = Kd @ x - lam * (Md @ x)
+        assert np.linalg.norm(res) <= 1e-7 * np.linalg.norm(Kd @ x)
+
+
+def test_modal_closed_form_frequencies(ksh):
+    """omega_mn = pi^2 (m^2 + n^2) sqrt(D / (rho t)) of the simply supported unit plate."""
+    t, E, nu, rho = 0.01, 1e7, 0.3, 1e3
+    pt = g.simply_supported_plate_problem(p=3, n=16, t=t, E=E, nu=nu, rho=rho)
+    r = _sp(pt).run_modal(4, tol=1e-12)
+    D = E * t ** 3 / (12 * (1 - nu ** 2))
+    w2 = np.sort([(np.pi ** 2 * (m * m + n * n)) ** 2 * D / (rho * t) for m in (1, 2, 3) for n in (1, 2, 3)])[:4]
+    assert np.allclose(r["values"], w2, rtol=2e-3)
+
+
+@pytest.mark.parametrize("precision", [L.FP64, L.FP32_IR])
+def test_buckling_matches_dense_oracle(precision, ksh):
+    """gsBucklingSolver(K_L, K_NL): A = K_L, B = K_NL - K_L, K_NL at u = K_L^-1 F (the C0 plate
+    with a small sinusoidal initial deflection)."""
+    z0 = lambda x, y: 0.05 * np.sin(np.pi * x) * np.sin(np.pi * y)
+    pt = g.clamped_plate_problem(p=3, n=8, z0=z0)
+    o = Oracle(pt)
+    print(f"[case] buckling clamped plate p=3 n=8 nfree={o.nfree} precision={precision}", flush=True)
+    KL = band_to_dense(o.tangent_band(np.zeros(o.nfree)), o.nfree, o.bw)
+    u = np.linalg.solve(KL, o.force())
+    KNL = band_to_dense(o.tangent_band(u), o.nfree, o.bw)
+    mu = sl.eigh(KNL - KL, KL, eigvals_only=True)  # (K_NL - K_L) x = mu K_L x, lambda = 1/mu
+    lam = 1.0 / mu[np.argsort(-np.abs(mu))[:8]]
+    ref = np.sort(lam)
+    r = _sp(pt).run_buckling(8, precision=precision, tol=1e-12)
+    assert r["converged"] == 8
+    assert np.abs(r["values"] - ref).max() <= 1e-8 * np.abs(ref).max()
