@@ -41,7 +41,7 @@ class AlmOptions(C.Structure):
                 ("relaxation", C.c_double), ("quasi", C.c_int), ("quasi_iterations", C.c_int),
                 ("adaptive", C.c_int), ("adaptive_iterations", C.c_int), ("max_refine", C.c_int),
                 ("refine_tol", C.c_double), ("max_halvings", C.c_int), ("verbose", C.c_int),
-                ("inertia_guard", C.c_double)]
+                ("inertia_guard", C.c_double), ("learned", C.c_void_p)]
 
 
 class AlmResult(C.Structure):
@@ -49,7 +49,8 @@ class AlmResult(C.Structure):
                 ("lambda_", _dp), ("unorm", _dp), ("indicator", _dp), ("negatives", _i64p),
                 ("iterations", _ip), ("residue_f", _dp), ("length", _dp), ("U", _dp), ("w", _dp),
                 ("probe_dof", C.c_int), ("wall_seconds", C.c_double), ("phi", C.c_double),
-                ("jacobians", C.c_int64), ("residuals", C.c_int64), ("solves", C.c_int64)]
+                ("jacobians", C.c_int64), ("residuals", C.c_int64), ("solves", C.c_int64),
+                ("features", _dp), ("learned_steps", C.c_int64), ("fallbacks", C.c_int64)]
 
 
 class StaticOptions(C.Structure):
@@ -128,6 +129,10 @@ SIGNATURES = {
     "ksh_mlp_forward": (C.c_int, [_VP, _fp, C.c_int, _fp]),
     "ksh_mlp_gradients": (C.c_int, [_VP, _fp, _fp, C.c_int, _fp, _dp]),
     "ksh_mlp_train": (C.c_int, [_VP, _fp, _fp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_uint64, _dp]),
+    "ksh_mlp_set_normalization": (C.c_int, [_VP, _dp, _dp, _dp, _dp]),
+    "ksh_mlp_predict": (C.c_int, [_VP, _dp, C.c_int, _dp]),
+    "ksh_predictor_features": (C.c_int, [_VP, _dp, C.c_double, _dp]),
+    "ksh_predictor_decode": (C.c_int, [_VP, _dp, C.c_double, _dp]),
 }
 
 _lib = None

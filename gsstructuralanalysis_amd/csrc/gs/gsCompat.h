@@ -157,6 +157,8 @@ struct gsExec {
     virtual int inertia(ksh_matrix *K, int64_t *neg, double *dmin) { return ksh_inertia(K, neg, dmin); }
     virtual int matvec(const ksh_matrix *K, const double *x, double *y) { return ksh_matvec(K, x, y); }
     virtual int vector_d(ksh_matrix *K, double *d) { return ksh_vector_d(K, d); }
+    // learned predictor network, one sample (x: network inputs, y: outputs)
+    virtual int predict(ksh_mlp *m, const double *x, double *y) { return ksh_mlp_predict(m, x, 1, y); }
     static gsExec *direct()
     {
         static gsExec e;

@@ -263,6 +263,7 @@ int ksh_problem_create(const ksh_patch *patch, const ksh_material *mat, const ks
     }
     P.nfree = (int)free2full.size();
     if (P.nfree == 0) return bail(fail(KSH_OTHER_ERROR, "no free dofs"));
+    P.h_free2full = free2full;
     int bw = 0;
     for (int ey = 0; ey < P.ny; ++ey)
         for (int ex = 0; ex < P.nx; ++ex) {

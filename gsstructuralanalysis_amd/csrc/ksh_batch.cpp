@@ -30,7 +30,7 @@ int alm_driver(ksh_problem *pb, gismo::gsExec *exec, const ksh_alm_options *o, k
 
 namespace {
 
-enum ReqKind { RQ_TANGENT = 0, RQ_RESIDUAL, RQ_FACTOR, RQ_SOLVE, RQ_INERTIA, RQ_OTHER };
+enum ReqKind { RQ_TANGENT = 0, RQ_RESIDUAL, RQ_FACTOR, RQ_SOLVE, RQ_INERTIA, RQ_PREDICT, RQ_KINDS };
 
 struct Req {
     ReqKind kind;
@@ -44,6 +44,7 @@ struct Req {
     double tol = 0.0;
     int64_t *neg = nullptr;
     double *dmin = nullptr;
+    ksh_mlp *net = nullptr;
     int rc = 0;
     bool done = false;
     int waited = 0;  // engine rounds this request has been left pending
@@ -154,7 +155,7 @@ private:
     void select(std::vector<Req *> &batch)
     {
         constexpr int kMaxWait = 16;
-        int cnt[6] = {0, 0, 0, 0, 0, 0};
+        int cnt[RQ_KINDS] = {};
         int forced = -1, oldest = -1;
         for (Req *r : m_pending) {
             ++cnt[r->kind];
@@ -166,7 +167,7 @@ private:
         int kind = forced;
         if (kind < 0) {
             kind = RQ_TANGENT;
-            for (int k = 0; k < 6; ++k)
+            for (int k = 0; k < RQ_KINDS; ++k)
                 if (k != RQ_INERTIA && cnt[k] > cnt[kind]) kind = k;
             if (cnt[kind] == 0) kind = RQ_INERTIA;
         }
@@ -184,7 +185,7 @@ private:
 
     int execute(std::vector<Req *> &batch)
     {
-        std::vector<Req *> by[6];
+        std::vector<Req *> by[RQ_KINDS];
         for (Req *r : batch) by[r->kind].push_back(r);
         int rc;
         if (!by[RQ_TANGENT].empty() && (rc = do_tangent(by[RQ_TANGENT]))) return rc;
@@ -192,6 +193,7 @@ private:
         if (!by[RQ_FACTOR].empty() && (rc = do_factor(by[RQ_FACTOR]))) return rc;
         if (!by[RQ_SOLVE].empty() && (rc = do_solve(by[RQ_SOLVE]))) return rc;
         for (Req *r : by[RQ_INERTIA]) r->rc = ksh_inertia(r->K, r->neg, r->dmin);
+        if (!by[RQ_PREDICT].empty() && (rc = do_predict(by[RQ_PREDICT]))) return rc;
         KSH_HIP(hipStreamSynchronize(m_stream));
         return KSH_SUCCESS;
     }
@@ -293,6 +295,27 @@ private:
                     g[i]->rc = fail(KSH_ASSEMBLY_ERROR, "non-finite residual");
                     break;
                 }
+        }
+        return KSH_SUCCESS;
+    }
+
+    // learned-predictor inference: one batched forward per network over the waiting instances
+    int do_predict(std::vector<Req *> &g)
+    {
+        constexpr int kIn = 264, kOut = 65;
+        std::vector<Req *> rest = g;
+        while (!rest.empty()) {
+            ksh_mlp *net = rest[0]->net;
+            std::vector<Req *> grp, other;
+            for (Req *r : rest) (r->net == net ? grp : other).push_back(r);
+            std::vector<double> X((size_t)grp.size() * kIn), Y((size_t)grp.size() * kOut);
+            for (size_t i = 0; i < grp.size(); ++i) std::memcpy(X.data() + i * kIn, grp[i]->in, sizeof(double) * kIn);
+            const int rc = ksh_mlp_predict(net, X.data(), (int)grp.size(), Y.data());
+            for (size_t i = 0; i < grp.size(); ++i) {
+                if (rc == KSH_SUCCESS) std::memcpy(grp[i]->out, Y.data() + i * kOut, sizeof(double) * kOut);
+                grp[i]->rc = rc;
+            }
+            rest.swap(other);
         }
         return KSH_SUCCESS;
     }
@@ -525,6 +548,14 @@ public:
         r.K = K;
         r.neg = neg;
         r.dmin = dmin;
+        return m_e->submit(r);
+    }
+    int predict(ksh_mlp *m, const double *x, double *y) override
+    {
+        Req r{RQ_PREDICT};
+        r.net = m;
+        r.in = x;
+        r.out = y;
         return m_e->submit(r);
     }
 

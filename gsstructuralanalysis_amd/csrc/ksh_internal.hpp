@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -75,6 +76,8 @@ struct Problem {
     int device = 0;
     hipStream_t stream = nullptr;
     double inertia_guard = 1e-6;   // guard given to the matrices created for this problem
+    std::vector<int> h_free2full;  // host copy of free2full
+    std::shared_ptr<void> modal;   // learned-predictor modal basis (built on first use)
 
     double *ctrl0 = nullptr;
     int *free_idx = nullptr;

@@ -177,6 +177,8 @@ struct Mlp {
     float *data_x = nullptr, *data_y = nullptr;
     int ndata = 0;
     long long adam_t = 0;
+    // normalization of the predictor's features / targets (identity until set)
+    std::vector<double> in_mean, in_std, out_mean, out_std;
 
     float *W(int l) { return param + woff[l]; }
     float *b(int l) { return param + boff[l]; }
@@ -377,6 +379,39 @@ int ksh_mlp_forward(ksh_mlp *m, const float *x, int batch, float *y)
     KSH_HIP(hipMemcpyAsync(y, m->H[Mlp::kLayers], sizeof(float) * (size_t)batch * m->width[Mlp::kLayers],
                            hipMemcpyDeviceToHost, m->s));
     KSH_HIP(hipStreamSynchronize(m->s));
+    return KSH_SUCCESS;
+}
+
+int ksh_mlp_set_normalization(ksh_mlp *m, const double *in_mean, const double *in_std, const double *out_mean,
+                              const double *out_std)
+{
+    if (!m || !in_mean || !in_std || !out_mean || !out_std) return fail(KSH_OTHER_ERROR, "null argument");
+    const int in = m->width[0], out = m->width[Mlp::kLayers];
+    m->in_mean.assign(in_mean, in_mean + in);
+    m->in_std.assign(in_std, in_std + in);
+    m->out_mean.assign(out_mean, out_mean + out);
+    m->out_std.assign(out_std, out_std + out);
+    return KSH_SUCCESS;
+}
+
+int ksh_mlp_predict(ksh_mlp *m, const double *x, int batch, double *y)
+{
+    if (!m || !x || !y || batch <= 0) return fail(KSH_OTHER_ERROR, "bad argument");
+    const int in = m->width[0], out = m->width[Mlp::kLayers];
+    const bool norm = !m->in_mean.empty();
+    std::vector<float> xf((size_t)batch * in), yf((size_t)batch * out);
+    for (int b = 0; b < batch; ++b)
+        for (int i = 0; i < in; ++i) {
+            const double v = x[(size_t)b * in + i];
+            xf[(size_t)b * in + i] = (float)(norm ? (v - m->in_mean[i]) / m->in_std[i] : v);
+        }
+    int rc;
+    if ((rc = ksh_mlp_forward(m, xf.data(), batch, yf.data()))) return rc;
+    for (int b = 0; b < batch; ++b)
+        for (int o = 0; o < out; ++o) {
+            const double v = yf[(size_t)b * out + o];
+            y[(size_t)b * out + o] = norm ? v * m->out_std[o] + m->out_mean[o] : v;
+        }
     return KSH_SUCCESS;
 }
 

@@ -11,6 +11,7 @@
 #include <exception>
 
 #include "gs/gsALMCrisfield.h"
+#include "gs/gsALMCrisfieldLearned.h"
 #include "gs/gsALMRiks.h"
 #include "gs/gsEigenSolvers.h"
 #include "gs/gsStaticNewton.h"
@@ -46,7 +47,11 @@ int alm_driver(ksh_problem *pb, gsExec *exec, const ksh_alm_options *o, ksh_alm_
         auto ALResidual = ops.alResidual();
 
         std::unique_ptr<gsALMBase<real_t>> alm;
-        if (o->method == 0) alm.reset(new gsALMCrisfield<real_t>(Jacobian, ALResidual, Force));
+        gsALMCrisfieldLearned<real_t> *learned = nullptr;
+        if (o->method == 0 && o->learned) {
+            learned = new gsALMCrisfieldLearned<real_t>(Jacobian, ALResidual, Force, pb, o->learned, exec);
+            alm.reset(learned);
+        } else if (o->method == 0) alm.reset(new gsALMCrisfield<real_t>(Jacobian, ALResidual, Force));
         else if (o->method == 1) alm.reset(new gsALMRiks<real_t>(Jacobian, ALResidual, Force));
         else return ksh::fail(KSH_OTHER_ERROR, "unknown arc-length method");
 
@@ -80,8 +85,21 @@ int alm_driver(ksh_problem *pb, gsExec *exec, const ksh_alm_options *o, ksh_alm_
         res->steps_done = 0;
         res->halvings = 0;
         res->final_status = 0;
+        res->learned_steps = 0;
+        res->fallbacks = 0;
         for (int k = 0; k < o->nsteps; ++k) {
             gsStatus status = alm->step();
+            if (learned && learned->usedLearned()) {
+                ++res->learned_steps;
+                if (status != gsStatus::Success) {
+                    // the corrector rejected the learned proposal: redo the step from the tangent predictor
+                    ++res->fallbacks;
+                    alm->setSolution(Uold, Lold);
+                    learned->setLearned(false);
+                    status = alm->step();
+                    learned->setLearned(true);
+                }
+            }
             res->final_status = (int)status;
             if (status == gsStatus::NotConverged || status == gsStatus::AssemblyError) {
                 if (res->halvings >= o->max_halvings) break;
@@ -96,6 +114,9 @@ int alm_driver(ksh_problem *pb, gsExec *exec, const ksh_alm_options *o, ksh_alm_
             if (status != gsStatus::Success) break;
             Uold = alm->solutionU();
             Lold = alm->solutionL();
+            if (learned) learned->record();
+            if (res->features && ksh_predictor_features(pb, Uold.data(), Lold, res->features + (size_t)k * 66) != KSH_SUCCESS)
+                return ksh::fail(KSH_OTHER_ERROR, ksh_last_error());
             if (res->lambda) res->lambda[k] = Lold;
             if (res->unorm) res->unorm[k] = Uold.norm();
             if (res->indicator) res->indicator[k] = alm->indicator();

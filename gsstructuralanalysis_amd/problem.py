@@ -29,15 +29,19 @@ def _f64(a):
 def alm_options(nsteps, length, method=0, precision=L.FP64, max_iter=20, tol=1e-6, tol_u=1e-6, tol_f=1e-3,
                 scaling=-1.0, angle_method=0, bifurcation_method=0, relaxation=1.0, quasi=0, quasi_iterations=-1,
                 adaptive=0, adaptive_iterations=10, max_refine=0, refine_tol=1e-14, max_halvings=10, verbose=0,
-                inertia_guard=0.0):
-    """ksh_alm_options with the reference's option names in snake case (gsALMBase.hpp:22-54)."""
+                inertia_guard=0.0, learned=None):
+    """ksh_alm_options with the reference's option names in snake case (gsALMBase.hpp:22-54);
+    `learned` is a LearnedPredictor (Crisfield only)."""
     return L.AlmOptions(method, precision, nsteps, length, max_iter, tol, tol_u, tol_f, scaling, angle_method,
                         bifurcation_method, relaxation, quasi, quasi_iterations, adaptive, adaptive_iterations,
-                        max_refine, refine_tol, max_halvings, verbose, inertia_guard)
+                        max_refine, refine_tol, max_halvings, verbose, inertia_guard,
+                        learned.handle if learned is not None else None)
 
 
-def _alm_result(nsteps, nfree, keep_U, probe_dof):
+def _alm_result(nsteps, nfree, keep_U, probe_dof, features=False):
     out = {k: np.zeros(nsteps) for k in ("lambda", "unorm", "indicator", "residue_f", "length", "w")}
+    if features:
+        out["features"] = np.zeros((nsteps, 66))
     out["negatives"] = np.zeros(nsteps, dtype=np.int64)
     out["iterations"] = np.zeros(nsteps, dtype=np.int32)
     U = np.zeros(nsteps * nfree) if keep_U else None
@@ -52,6 +56,7 @@ def _alm_result(nsteps, nfree, keep_U, probe_dof):
     r.w = _dptr(out["w"])
     r.probe_dof = probe_dof
     r.U = _dptr(U) if U is not None else None
+    r.features = _dptr(out["features"]) if features else None
     return r, out, U
 
 
@@ -59,13 +64,14 @@ def _alm_collect(r, out, U, nsteps, nfree):
     k = r.steps_done
     res = {key: val[:k] for key, val in out.items()}
     res.update(steps_done=k, halvings=r.halvings, final_status=r.final_status, wall_seconds=r.wall_seconds,
-               phi=r.phi, jacobians=r.jacobians, residuals=r.residuals, solves=r.solves)
+               phi=r.phi, jacobians=r.jacobians, residuals=r.residuals, solves=r.solves,
+               learned_steps=r.learned_steps, fallbacks=r.fallbacks)
     if U is not None:
         res["U"] = U.reshape(nsteps, nfree)[:k]
     return res
 
 
-def run_alm_batch(problems, nsteps, length, keep_U=False, **kw):
+def run_alm_batch(problems, nsteps, length, keep_U=False, features=False, **kw):
     """Arc-length paths of many problems at once (ksh_batch_run_alm): each runs the unchanged
     solver loop in its own host thread; device calls are merged into batched launches.
     Returns (list of per-problem result dicts, wall seconds of the whole batch)."""
@@ -75,7 +81,7 @@ def run_alm_batch(problems, nsteps, length, keep_U=False, **kw):
     rs = (L.AlmResult * n)()
     keep = []
     for i, p in enumerate(problems):
-        r, out, U = _alm_result(nsteps, p.nfree, keep_U, -1)
+        r, out, U = _alm_result(nsteps, p.nfree, keep_U, -1, features)
         rs[i] = r
         keep.append((out, U))
     wall = C.c_double()
@@ -226,9 +232,9 @@ class ShellProblem:
         return M
 
     # -- drivers ---------------------------------------------------------------------------
-    def run_alm(self, nsteps, length, keep_U=False, probe_dof=-1, **kw):
+    def run_alm(self, nsteps, length, keep_U=False, probe_dof=-1, features=False, **kw):
         o = alm_options(nsteps, length, **kw)
-        r, out, U = _alm_result(nsteps, self.nfree, keep_U, probe_dof)
+        r, out, U = _alm_result(nsteps, self.nfree, keep_U, probe_dof, features)
         code = L.load().ksh_run_alm(self.handle, C.byref(o), C.byref(r))
         if code not in (L.SUCCESS, L.NOT_CONVERGED):
             L.check(code)

@@ -52,6 +52,7 @@ enum ksh_precision {
 
 typedef struct ksh_problem ksh_problem;
 typedef struct ksh_matrix ksh_matrix;
+typedef struct ksh_mlp ksh_mlp;
 
 typedef struct {
     int p;              /* degree in both directions (2..4)                    */
@@ -177,6 +178,9 @@ typedef struct {
     double inertia_guard;    /* FP32 factors: a factor whose min |d| < guard max |d| is   */
                              /* replaced by an FP64 one for its solves and pivot count    */
                              /* (0: the library default, 1e-6)                            */
+    struct ksh_mlp *learned; /* Crisfield only: learned predictor network (NULL: tangent   */
+                             /* predictor); a step that fails from the learned proposal   */
+                             /* is redone from the tangent predictor                      */
 } ksh_alm_options;
 
 typedef struct {
@@ -196,6 +200,9 @@ typedef struct {
     double wall_seconds;
     double phi;              /* Crisfield scaling in use at the end                       */
     int64_t jacobians, residuals, solves;
+    double *features;        /* [nsteps*66] predictor features of each converged state (opt.) */
+    int64_t learned_steps;   /* steps that started from the learned proposal              */
+    int64_t fallbacks;       /* of those, steps redone from the tangent predictor         */
 } ksh_alm_result;
 
 /* benchmarks/benchmark_Roof.cpp:320-512: options, initialize, step loop with halving. */
@@ -289,7 +296,6 @@ int ksh_run_modal(ksh_problem *pb, const ksh_eig_options *opt, double *values, d
  * widths[4] = {264, 256, 256, 65}: GELU hidden layers, linear output; BF16 products with FP32
  * accumulation on MFMA, FP32 master weights, Adam.  Parameters are packed as W0, W1, W2
  * (row-major [in][out]) then b0, b1, b2.  Host arrays, row-major samples. */
-typedef struct ksh_mlp ksh_mlp;
 int ksh_mlp_create(const int *widths, uint64_t seed, ksh_mlp **out);
 int ksh_mlp_destroy(ksh_mlp *m);
 int ksh_mlp_num_params(const ksh_mlp *m, int64_t *n);
@@ -301,6 +307,18 @@ int ksh_mlp_gradients(ksh_mlp *m, const float *x, const float *target, int batch
 /* minibatch Adam over the samples (full batches per epoch, shuffled with `seed`) */
 int ksh_mlp_train(ksh_mlp *m, const float *x, const float *target, int nsamples, int epochs, int batch, double lr,
                   double weight_decay, uint64_t seed, double *loss_per_epoch);
+/* input / output normalization used by ksh_mlp_predict: x_n = (x - in_mean) / in_std,
+ * y = y_n out_std + out_mean */
+int ksh_mlp_set_normalization(ksh_mlp *m, const double *in_mean, const double *in_std, const double *out_mean,
+                              const double *out_std);
+int ksh_mlp_predict(ksh_mlp *m, const double *x, int batch, double *y);
+
+/* The predictor's 66 features of a converged state (U on the free dofs, lambda): lambda, ||U||,
+ * and the 64 least-squares coefficients of the out-of-plane displacement in the modes
+ * sin(a pi xi) sin(b pi eta), a, b = 1..8, at the Greville points of the control net. */
+int ksh_predictor_features(ksh_problem *pb, const double *U, double lambda, double *out66);
+/* dU[free] += scale * (out-of-plane field of the 64 mode coefficients coef64) */
+int ksh_predictor_decode(ksh_problem *pb, const double *coef64, double scale, double *dU);
 
 #ifdef __cplusplus
 }
