@@ -199,6 +199,13 @@ int inertia_fp64(Matrix &K, int64_t *neg, double *dmin)
 
 int refactor_fp64(Matrix &K) { return inertia_fp64(K, nullptr, nullptr); }
 
+int ensure_fac64(Matrix &K)
+{
+    int rc = 0;
+    if (!K.fac64 && !(K.fac64 = dalloc<double>((size_t)K.n * K.ld, rc))) return rc;
+    return KSH_SUCCESS;
+}
+
 }  // namespace ksh
 
 using namespace ksh;
@@ -441,7 +448,7 @@ int ksh_matrix_create(ksh_problem *pb, int prec, ksh_matrix **out)
     };
     if (!(K->ab = dalloc<unsigned char>(count * eb, rc))) return bail(rc);
     if (!(K->fac = dalloc<unsigned char>(count * fb, rc))) return bail(rc);
-    if (!(K->work = dalloc<double>((size_t)4 * K->n, rc))) return bail(rc);
+    if (!(K->work = dalloc<double>((size_t)5 * K->n, rc))) return bail(rc);
     if (!(K->flag = dalloc<int>(4, rc))) return bail(rc);
     if (!(K->red = dalloc<double>(8, rc))) return bail(rc);
     if (K->sprec == KSH_FP32 && !(K->u_at = dalloc<double>(K->n, rc))) return bail(rc);

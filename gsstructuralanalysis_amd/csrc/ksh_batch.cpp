@@ -362,16 +362,65 @@ private:
                 K.info = info[k];
                 for (int q = 0; q < 4; ++q) K.fstats[q] = st[4 * k + q];
                 K.has_stats = info[k] == 0;
-                bool fallback = fp == KSH_FP32 && (info[k] != 0 || !(K.fstats[3] >= K.guard * K.fstats[2]));
-                if (fallback) {
-                    // the problem's own stream must see the engine's writes
-                    KSH_HIP(hipStreamSynchronize(m_stream));
-                    int r2 = refactor_fp64(K);
-                    K.info = r2 == KSH_SUCCESS ? 0 : 1;
-                }
-                grp[k]->rc = K.info ? fail(KSH_SOLVER_ERROR, "LDL^T: zero or non-finite pivot") : KSH_SUCCESS;
+            }
+            // FP32 factors with a pivot too small to trust (or a failed one): FP64 refactorization
+            // of the same matrices, all in one batched launch
+            std::vector<Matrix *> re;
+            for (size_t k = 0; k < grp.size(); ++k) {
+                Matrix &K = *grp[k]->K;
+                if (fp == KSH_FP32 && (info[k] != 0 || !(K.fstats[3] >= K.guard * K.fstats[2]))) re.push_back(&K);
+            }
+            if ((rc = refactor64_batch(re))) return rc;
+            for (size_t k = 0; k < grp.size(); ++k)
+                grp[k]->rc = grp[k]->K->info ? fail(KSH_SOLVER_ERROR, "LDL^T: zero or non-finite pivot") : KSH_SUCCESS;
+        }
+        return KSH_SUCCESS;
+    }
+
+    // FP64 refactorization of several matrices whose FP32 factor cannot be trusted, in one
+    // batched launch; per matrix the same result as refactor_fp64 (an FP64 factor of the
+    // assembled values, re-assembled in FP64 from the stored point for FP32 storage), and the
+    // solves and pivot count of the current version use it.
+    int refactor64_batch(const std::vector<Matrix *> &ms)
+    {
+        if (ms.empty()) return KSH_SUCCESS;
+        int rc;
+        if ((rc = ensure_desc(ms.size()))) return rc;
+        std::vector<InstDesc> tang;
+        for (Matrix *K : ms) {
+            const size_t count = (size_t)K->n * K->ld;
+            if ((rc = ensure_fac64(*K))) return rc;
+            if (K->sprec == KSH_FP64) {
+                KSH_HIP(hipMemcpyAsync(K->fac64, K->ab, count * sizeof(double), hipMemcpyDeviceToDevice, m_stream));
+            } else if (K->has_u) {
+                InstDesc d = make_desc(*K->pb);
+                d.u_free = K->u_at;
+                d.ab = K->fac64;
+                d.sprec = KSH_FP64;
+                tang.push_back(d);
+            } else if ((rc = band_convert(K->ab, K->sprec, K->fac64, KSH_FP64, count, m_stream))) {
+                return rc;
             }
         }
+        if (!tang.empty() && (rc = batch_tangent(m_stream, (InstDesc *)m_desc.p, tang))) return rc;
+        std::vector<BandDescHost> h;
+        for (size_t k = 0; k < ms.size(); ++k)
+            h.push_back(BandDescHost{ms[k]->fac64, ms[k]->n, ms[k]->bw, (int *)m_info.p + k, (double *)m_stats.p + 4 * k, nullptr, 1});
+        if ((rc = band_factor_batch(m_stream, m_bdesc.p, h, KSH_FP64))) return rc;
+        std::vector<int> info(ms.size());
+        std::vector<double> st(4 * ms.size());
+        KSH_HIP(hipMemcpyAsync(info.data(), m_info.p, sizeof(int) * ms.size(), hipMemcpyDeviceToHost, m_stream));
+        KSH_HIP(hipMemcpyAsync(st.data(), m_stats.p, sizeof(double) * 4 * ms.size(), hipMemcpyDeviceToHost, m_stream));
+        KSH_HIP(hipStreamSynchronize(m_stream));
+        for (size_t k = 0; k < ms.size(); ++k) {
+            Matrix &K = *ms[k];
+            K.use64 = true;
+            K.info = info[k] == 0 ? 0 : 1;
+            for (int q = 0; q < 4; ++q) K.fstats[q] = st[4 * k + q];
+            K.has_stats = info[k] == 0;
+        }
+        ++m_refactor_launches;
+        m_refactors += (long long)ms.size();
         return KSH_SUCCESS;
     }
 
@@ -462,14 +511,20 @@ private:
             for (size_t i = 0; i < g.size(); ++i)
                 if (last[i]) active[i] = 0;
         }
-        // FP64 fallback of stalled refinements: x = K64^-1 b
+        // FP64 fallback of stalled refinements: x = K64^-1 b (one batched refactorization)
         {
+            std::vector<Matrix *> re;
+            for (size_t i = 0; i < g.size(); ++i)
+                if (fallback[i]) re.push_back(g[i]->K);
+            if ((rc = refactor64_batch(re))) return rc;
             std::vector<BandDescHost> h64;
             for (size_t i = 0; i < g.size(); ++i) {
                 if (!fallback[i]) continue;
                 Matrix &K = *g[i]->K;
-                KSH_HIP(hipStreamSynchronize(m_stream));  // the problem's stream reads the engine's K
-                if ((rc = refactor_fp64(K))) return rc;
+                if (K.info) {
+                    err[i] = 1;
+                    continue;
+                }
                 KSH_HIP(hipMemcpyAsync(x + off[i], b + off[i], sizeof(double) * K.n, hipMemcpyDeviceToDevice, m_stream));
                 h64.push_back(BandDescHost{K.fac64, K.n, K.bw, nullptr, nullptr, x + off[i], 1});
             }
@@ -495,6 +550,7 @@ private:
     std::condition_variable m_cv_main, m_cv_done;
     int m_active = 0, m_blocked = 0;
     long long m_rounds = 0;
+    long long m_refactors = 0, m_refactor_launches = 0;  // batched FP64 refactorizations (matrices, launches)
     std::vector<Req *> m_pending;
     hipStream_t m_stream = nullptr;
     DevBuf m_din[3], m_dout, m_dr, m_desc, m_bdesc, m_rdesc, m_stats, m_info, m_norms, m_part;

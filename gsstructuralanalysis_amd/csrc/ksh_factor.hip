@@ -29,6 +29,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <string>
 
 #include "ksh_devutil.hpp"
@@ -384,14 +386,29 @@ int check_factor_shape(int n, int bw, int prec, int &nb)
     return KSH_SUCCESS;
 }
 
+// single bands of at least this many multiply-adds (n bw^2) are factored over the whole GPU
+constexpr double kTiledFactorWork = 1e30;
+
 }  // namespace
+
+// KSH_BAND_PATH=tiled / wg forces the multi-workgroup / one-workgroup path where both apply
+bool band_use_tiled_factor(int n, int bw, int prec)
+{
+    const char *e = std::getenv("KSH_BAND_PATH");
+    if (e && std::strcmp(e, "tiled") == 0) return true;
+    const int nb = prec == KSH_FP64 ? panel_width<double>(bw) : panel_width<float>(bw);
+    if (nb == 0) return true;
+    if (e && std::strcmp(e, "wg") == 0) return false;
+    return (double)n * bw * bw >= kTiledFactorWork;
+}
 
 int band_factor(Matrix &K, void *fac, int prec, double *stats_out)
 {
     Problem &pb = *K.pb;
     const size_t bytes = (size_t)K.n * K.ld * (prec == KSH_FP64 ? sizeof(double) : sizeof(float));
-    int nb, rc;
-    if ((rc = check_factor_shape(K.n, K.bw, prec, nb))) return rc;
+    const bool tiled = band_use_tiled_factor(K.n, K.bw, prec);
+    int nb = 0, rc;
+    if (!tiled && (rc = check_factor_shape(K.n, K.bw, prec, nb))) return rc;
     if (fac != K.ab) {
         if (prec == K.sprec) {
             KSH_HIP(hipMemcpyAsync(fac, K.ab, bytes, hipMemcpyDeviceToDevice, pb.stream));
@@ -400,7 +417,11 @@ int band_factor(Matrix &K, void *fac, int prec, double *stats_out)
         }
     }
     double *dstats = K.red + 4;
-    {
+    if (tiled) {
+        ProfScope ps("band_ldlt_tiled", pb.stream);
+        ps.set_work((double)K.n * ((double)K.bw * K.bw + 3.0 * K.bw));
+        rc = band_factor_tiled(pb.stream, fac, prec, K.n, K.bw, K.flag, dstats);
+    } else {
         ProfScope ps("band_ldlt", pb.stream);
         ps.set_work((double)K.n * ((double)K.bw * K.bw + 3.0 * K.bw));
         rc = prec == KSH_FP64 ? launch_factor<double>(pb.stream, nb, 1, nullptr, (double *)fac, K.n, K.bw, K.flag, dstats)

@@ -1,0 +1,260 @@
+// ksh_factor_tiled.hip -- banded LDL^T factorization spread over the whole GPU, for one large
+// band (wide bands whose panel does not fit one workgroup's LDS, or long ones where a single
+// workgroup would leave the other CUs idle).
+//
+// Same algorithm and layout as the one-workgroup factor (ksh_factor.hip): lower band, column
+// major, fac[j*ld + (i-j)], ld = bw+1, right-looking in panels of NB = 32 columns, no pivoting
+// (SimplicialLDLT::compute on the eigen, Newton and arc-length paths,
+// src/gsEigenSolvers/gsBucklingSolver.h:131-133, src/gsALMSolvers/gsALMBase.hpp:186-211).
+// Each panel step is two launches on the stream:
+//   1. k_band_panel (one workgroup): wave 0 factors the NB x NB diagonal block in registers
+//      (lane = row, v_readlane broadcasts); every thread then solves one row below it against
+//      L_dd D (row-wise TRSM) straight from HBM, so no panel has to fit LDS;
+//   2. k_band_trailing (a workgroup of 4 waves per 4 blocks): the lower triangle of the
+//      bw x bw trailing window is cut into 32 x 32 blocks, one per wave, each a 2 x 2 set of
+//      16 x 16 MFMA tiles (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32); the panel operands
+//      -d_k L[j][k] and L[i][k] are read from L2 (the panel was just written), the block is read,
+//      updated and written once.
+// Work per matrix: about n bw^2 multiply-adds; HBM traffic about 2 n bw^2 / NB entries.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+
+#include "ksh_devutil.hpp"
+#include "ksh_internal.hpp"
+#include "ksh_mfma.hpp"
+#include "ksh_profile.hpp"
+
+namespace ksh {
+
+namespace {
+
+constexpr int kNB = 32;       // panel width
+constexpr int kPT = 256;      // panel kernel threads
+constexpr int kUW = 4;        // waves per trailing-update workgroup
+
+template <typename T>
+__global__ __launch_bounds__(kPT) void k_band_panel(T *fac, int n, int bw, int c0, int *info)
+{
+    __shared__ T blk[kNB * kNB];  // blk[kk*NB + r] = A / L[c0+r][c0+kk] for r >= kk, zero above
+    __shared__ T dinv[kNB];
+    __shared__ T colv[2][kNB];    // column of the current elimination step (double-buffered)
+    const unsigned ldu = (unsigned)bw + 1;
+    constexpr unsigned es = sizeof(T);
+    const __amdgpu_buffer_rsrc_t rs = band_rsrc(fac, (unsigned)((size_t)n * ldu * es));
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int w = min(kNB, n - c0), nr = min(n - c0, w + bw);
+    constexpr int kQ = kNB * kNB / kPT;
+    // stage the diagonal block (unit pivots pad the last panel)
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        const int idx = tid + kPT * q, kk = idx / kNB, r = idx % kNB;
+        const bool ok = r >= kk && r < w && r - kk <= bw;
+        const T v = load_if<T>(ok, rs, ((unsigned)(c0 + kk) * ldu + (unsigned)(r - kk)) * es);
+        blk[idx] = (r >= w && r == kk) ? T(1) : v;
+    }
+    __syncthreads();
+    if (wave == 0) {
+        // lane r < NB owns row c0+r; lanes >= NB shadow row r-NB.  Step jj shares column jj
+        // through LDS (every lane reads the same entries: broadcasts).  Entries above the
+        // diagonal take updates too; they are never read.
+        const int r = lane & (kNB - 1);
+        T a[kNB];
+#pragma unroll
+        for (int kk = 0; kk < kNB; ++kk) a[kk] = blk[kk * kNB + r];
+        int bad = 0;
+#pragma unroll
+        for (int jj = 0; jj < kNB; ++jj) {
+            T *col = colv[jj & 1];
+            if (lane < kNB) col[r] = a[jj];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+            const T djj = col[jj];
+            if (jj < w && bad == 0 && (djj == T(0) || !isfinite((double)djj))) bad = c0 + jj + 1;
+            const T l = a[jj] / djj;
+#pragma unroll
+            for (int kk = jj + 1; kk < kNB; ++kk) a[kk] -= l * col[kk];
+            if (r > jj) a[jj] = l;
+            __builtin_amdgcn_sched_barrier(0);  // keeps the steps' LDS reads from being hoisted together
+        }
+        // first bad pivot of the factorization: the first panel clears the flag
+        if (lane == 0 && (c0 == 0 || (bad != 0 && *info == 0))) *info = bad;
+        if (lane < kNB) {
+            dinv[r] = T(1) / a[r];
+#pragma unroll
+            for (int kk = 0; kk < kNB; ++kk) blk[kk * kNB + r] = kk <= r ? a[kk] : T(0);
+        }
+    }
+    __syncthreads();
+    // write the factored block back (L below the diagonal, D on it)
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        const int idx = tid + kPT * q, kk = idx / kNB, r = idx % kNB;
+        const bool ok = r >= kk && r < w && r - kk <= bw;
+        store_if<T>(ok, rs, ((unsigned)(c0 + kk) * ldu + (unsigned)(r - kk)) * es, blk[idx]);
+    }
+    // TRSM: rows r = w .. nr-1 of the panel, L[r][0:NB] = A[r][0:NB] (D L_dd^T)^-1
+    for (int r = w + tid; r < nr; r += kPT) {
+        T a[kNB];
+#pragma unroll
+        for (int kk = 0; kk < kNB; ++kk) {
+            const bool ok = kk < w && r - kk <= bw;
+            a[kk] = load_if<T>(ok, rs, ((unsigned)(c0 + kk) * ldu + (unsigned)(r - kk)) * es);
+        }
+#pragma unroll
+        for (int kk = 0; kk < kNB; ++kk) {
+            const T t = a[kk];  // = L[r][kk] d_kk
+#pragma unroll
+            for (int jj = kk + 1; jj < kNB; ++jj) a[jj] -= t * blk[kk * kNB + jj];
+            a[kk] = t * dinv[kk];
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int kk = 0; kk < kNB; ++kk) {
+            const bool ok = kk < w && r - kk <= bw;
+            store_if<T>(ok, rs, ((unsigned)(c0 + kk) * ldu + (unsigned)(r - kk)) * es, a[kk]);
+        }
+    }
+}
+
+// lower-triangle block index p -> (bi, bj), bj <= bi
+__device__ __forceinline__ void tri_decode(int p, int &bi, int &bj)
+{
+    int b = (int)((sqrtf(8.0f * (float)p + 1.0f) - 1.0f) * 0.5f);
+    if ((b + 1) * (b + 2) / 2 <= p) ++b;
+    if (b * (b + 1) / 2 > p) --b;
+    bi = b;
+    bj = p - b * (b + 1) / 2;
+}
+
+// A[i][j] -= sum_k L[i][k] d_k L[j][k] over the window rows/columns [w, nr) of the frame at c0
+template <typename T>
+__global__ __launch_bounds__(64 * kUW) void k_band_trailing(T *fac, int n, int bw, int c0, int w, int nr, int nblock)
+{
+    using M = Mfma16<T>;
+    using acc_t = typename M::acc_t;
+    const unsigned ldu = (unsigned)bw + 1;
+    constexpr unsigned es = sizeof(T);
+    const __amdgpu_buffer_rsrc_t rs = band_rsrc(fac, (unsigned)((size_t)n * ldu * es));
+    const int lane = threadIdx.x & 63;
+    const int p = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kUW + (threadIdx.x >> 6)));
+    if (p >= nblock) return;
+    int bi, bj;
+    tri_decode(p, bi, bj);
+    const int li = lane & 15, lk = lane >> 4;
+    const bool diag = bi == bj;
+    // panel operands: rows rj (block column bj) and ri (block row bi), k = 4 s + lk
+    const int ri0 = w + 32 * bi + li, rj0 = w + 32 * bj + li;
+    T A0[kNB / 4], A1[kNB / 4], B0[kNB / 4], B1[kNB / 4];
+#pragma unroll
+    for (int s = 0; s < kNB / 4; ++s) {
+        const int k = 4 * s + lk;
+        const unsigned col = ((unsigned)c0 * ldu + (unsigned)k * (ldu - 1)) * es;  // + row * es
+        const T nd = -load_if<T>(k < w, rs, (unsigned)(c0 + k) * ldu * es);
+        A0[s] = nd * load_if<T>(k < w && rj0 < nr && rj0 - k <= bw, rs, col + (unsigned)rj0 * es);
+        A1[s] = nd * load_if<T>(k < w && rj0 + 16 < nr && rj0 + 16 - k <= bw, rs, col + (unsigned)(rj0 + 16) * es);
+        B0[s] = load_if<T>(k < w && ri0 < nr && ri0 - k <= bw, rs, col + (unsigned)ri0 * es);
+        B1[s] = load_if<T>(k < w && ri0 + 16 < nr && ri0 + 16 - k <= bw, rs, col + (unsigned)(ri0 + 16) * es);
+    }
+    // accumulator rows = j (A's column), columns = i (A's row): lanes run along a column
+    acc_t cur[2][2];
+    unsigned off[2][2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = w + 32 * bi + 16 * a + li, j = w + 32 * bj + 16 * b + M::row(lane, v);
+                const bool ok = i < nr && i >= j;
+                off[a][b][v] = ok ? ((unsigned)c0 * ldu + (unsigned)j * (ldu - 1) + (unsigned)i) * es : kMaskedOff;
+                cur[a][b][v] = load_vs<T>(rs, off[a][b][v], 0);
+            }
+#pragma unroll
+    for (int s = 0; s < kNB / 4; ++s) {
+        cur[0][0] = M::run(A0[s], B0[s], cur[0][0]);
+        cur[1][0] = M::run(A0[s], B1[s], cur[1][0]);
+        cur[1][1] = M::run(A1[s], B1[s], cur[1][1]);
+        if (!diag) cur[0][1] = M::run(A1[s], B0[s], cur[0][1]);
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) store_vs<T>(rs, off[a][b][v], 0, cur[a][b][v]);
+}
+
+// negatives, min d, max |d|, min |d| of the factored diagonal (one workgroup of 1024 threads)
+template <typename T>
+__global__ __launch_bounds__(1024) void k_pivot_stats(const T *fac, int n, int bw, double *stats)
+{
+    __shared__ double part[16][4];
+    const size_t ld = (size_t)bw + 1;
+    double c = 0.0, mn = INFINITY, mxa = 0.0, mna = INFINITY;
+    for (int i = threadIdx.x; i < n; i += 1024) {
+        const double d = (double)fac[(size_t)i * ld];
+        c += d < 0.0 ? 1.0 : 0.0;
+        mn = fmin(mn, d);
+        mxa = fmax(mxa, fabs(d));
+        mna = fmin(mna, fabs(d));
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        c += __shfl_xor(c, o);
+        mn = fmin(mn, __shfl_xor(mn, o));
+        mxa = fmax(mxa, __shfl_xor(mxa, o));
+        mna = fmin(mna, __shfl_xor(mna, o));
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        part[wave][0] = c;
+        part[wave][1] = mn;
+        part[wave][2] = mxa;
+        part[wave][3] = mna;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < 16; ++q) {
+            c += part[q][0];
+            mn = fmin(mn, part[q][1]);
+            mxa = fmax(mxa, part[q][2]);
+            mna = fmin(mna, part[q][3]);
+        }
+        stats[0] = c;
+        stats[1] = mn;
+        stats[2] = mxa;
+        stats[3] = mna;
+    }
+}
+
+template <typename T>
+int run_tiled(hipStream_t s, T *fac, int n, int bw, int *info, double *stats)
+{
+    for (int c0 = 0; c0 < n; c0 += kNB) {
+        const int w = std::min(kNB, n - c0), nr = std::min(n - c0, w + bw);
+        hipLaunchKernelGGL(k_band_panel<T>, dim3(1), dim3(kPT), 0, s, fac, n, bw, c0, info);
+        if (nr > w) {
+            const int nbk = (nr - w + 31) / 32, nblock = nbk * (nbk + 1) / 2;
+            hipLaunchKernelGGL(k_band_trailing<T>, dim3((unsigned)((nblock + kUW - 1) / kUW)), dim3(64 * kUW), 0, s, fac, n, bw,
+                               c0, w, nr, nblock);
+        }
+    }
+    if (stats) hipLaunchKernelGGL(k_pivot_stats<T>, dim3(1), dim3(1024), 0, s, (const T *)fac, n, bw, stats);
+    KSH_HIP(hipGetLastError());
+    return KSH_SUCCESS;
+}
+
+}  // namespace
+
+int band_factor_tiled(hipStream_t s, void *fac, int prec, int n, int bw, int *info, double *stats)
+{
+    const size_t es = prec == KSH_FP64 ? sizeof(double) : sizeof(float);
+    if ((size_t)n * (size_t)(bw + 1) * es >= 0x80000000ull)
+        return fail(KSH_SOLVER_ERROR, "band of 2 GiB or more: beyond the band kernels' 32-bit offsets");
+    if (n < 1) return fail(KSH_SOLVER_ERROR, "empty band");
+    return prec == KSH_FP64 ? run_tiled<double>(s, (double *)fac, n, bw, info, stats)
+                            : run_tiled<float>(s, (float *)fac, n, bw, info, stats);
+}
+
+}  // namespace ksh

@@ -117,7 +117,7 @@ struct Matrix {
     double fstats[4] = {0, 0, 0, 0};  // pivot statistics of the current factor (negatives, min d, max|d|, min|d|)
     bool has_stats = false;
     // device workspace for solves and inertia
-    double *work = nullptr;        // 4*n doubles
+    double *work = nullptr;        // 5*n doubles (the last n: tiled-solve workspace)
     int *flag = nullptr;           // small device int buffer
     double *red = nullptr;         // reduction scratch
 };
@@ -166,9 +166,18 @@ int refine_residual_batch(hipStream_t s, RefDesc *d_descs, const std::vector<Ref
 int refine_update_batch(hipStream_t s, RefDesc *d_descs, const std::vector<RefDesc> &h);
 // FP64 refactorization of K's assembled values into fac64 (K.use64 = true afterwards)
 int refactor_fp64(Matrix &K);
+// allocates K.fac64 if it is not there yet
+int ensure_fac64(Matrix &K);
 int band_solve_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHost> &h, int prec);
 constexpr size_t kBandDescBytes = 48;
 int band_solve(Matrix &K, const void *fac, int prec, double *x_dev);
+// one large band over the whole GPU (ksh_factor_tiled.hip, ksh_solve_tiled.hip): panel steps of
+// two launches; stats (optional) as band_factor's; the solve uses z (n doubles) as workspace
+int band_factor_tiled(hipStream_t s, void *fac, int prec, int n, int bw, int *info, double *stats);
+int band_solve_tiled(hipStream_t s, const void *fac, int prec, int n, int bw, double *x, double *z);
+// path of a single-band factor / solve: 1 = multi-workgroup (tiled), 0 = one workgroup
+bool band_use_tiled_factor(int n, int bw, int prec);
+bool band_use_tiled_solve(int n, int bw, int prec);
 int band_symv(const Matrix &K, const double *x_dev, double *y_dev);
 int band_inertia(Matrix &K, const void *fac, int prec, int64_t *neg, double *dmin, double *dmaxabs,
                  double *dminabs);

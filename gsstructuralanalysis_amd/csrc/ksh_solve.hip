@@ -24,6 +24,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 
 #include "ksh_devutil.hpp"
 #include "ksh_internal.hpp"
@@ -417,9 +419,27 @@ int band_solve_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHos
     return KSH_SUCCESS;
 }
 
+// KSH_BAND_PATH=tiled / wg forces the multi-workgroup / one-workgroup path.  Otherwise bands
+// that the LDS-resident kernel takes stay on one workgroup; wider ones go multi-workgroup when
+// a step launch (~4 us for 64 rows) costs less than one CU streaming those rows' band entries.
+bool band_use_tiled_solve(int n, int bw, int prec)
+{
+    const char *e = std::getenv("KSH_BAND_PATH");
+    if (e && std::strcmp(e, "tiled") == 0) return true;
+    if (e && std::strcmp(e, "wg") == 0) return false;
+    const size_t es = prec == KSH_FP64 ? sizeof(double) : sizeof(float);
+    if (solve_lds(n, bw, (size_t)n * (bw + 1) * es, prec)) return false;
+    return (double)(bw + 1) * es >= 3072.0;
+}
+
 int band_solve(Matrix &K, const void *fac, int prec, double *x_dev)
 {
     Problem &pb = *K.pb;
+    if (band_use_tiled_solve(K.n, K.bw, prec)) {
+        ProfScope ps("band_solve_tiled", pb.stream);
+        ps.set_work(2.0 * K.n * (K.bw + 1.0) * (prec == KSH_FP64 ? 8.0 : 4.0) + 16.0 * K.n);
+        return band_solve_tiled(pb.stream, fac, prec, K.n, K.bw, x_dev, K.work + 4 * (size_t)K.n);
+    }
     const size_t smem = solve_lds(K.n, K.bw, (size_t)K.n * K.ld * (prec == KSH_FP64 ? sizeof(double) : sizeof(float)), prec);
     ProfScope ps("band_solve", pb.stream);
     ps.set_work(2.0 * K.n * (K.bw + 1.0) * (prec == KSH_FP64 ? 8.0 : 4.0) + 16.0 * K.n);

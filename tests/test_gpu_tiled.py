@@ -1,0 +1,106 @@
+"""Multi-workgroup band LDL^T factor and solve (one large band over the whole GPU) against the
+oracle's band LDL^T: pivots (vectorD), negative-pivot counts and solutions.
+
+KSH_BAND_PATH=tiled forces the multi-workgroup path on bands the one-workgroup kernels also
+take, so the comparison covers panel edges (last panel narrower than 32 columns, frames cut by
+the matrix end) on small shapes; the widest case (bw 1298) takes it by itself: its 32- and
+16-column panels do not fit one workgroup's LDS.  Indefinite matrices are K - sigma M with sigma
+between two eigenvalues of the pencil, so the count is fixed by Sylvester's law.
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+from gsstructuralanalysis_amd import _lib as L
+from gsstructuralanalysis_amd import geometry as g
+from oracle import ok
+from oracle.ok import Oracle
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(2, 3, None), (3, 8, None), (3, 21, 7), (3, 64, 3)]
+
+
+def _shifted(o, u, k):
+    """K(u) - sigma M with sigma halfway between the k-th and (k+1)-th pencil eigenvalues."""
+    Kb = o.tangent_band(u)
+    Mb = o.mass_band()
+    Kd = ok.band_to_dense(Kb, o.nfree, o.bw)
+    Md = ok.band_to_dense(Mb, o.nfree, o.bw)
+    ev = sl.eigh(Kd, Md, eigvals_only=True, subset_by_index=[0, k])
+    sigma = 0.5 * (ev[k - 1] + ev[k])
+    return Kb - sigma * Mb
+
+
+@pytest.mark.parametrize("p,n,ny", CASES)
+@pytest.mark.parametrize("prec", [L.FP64, L.FP32_IR])
+def test_tiled_factor_and_solve_match_oracle(p, n, ny, prec, ksh, monkeypatch):
+    from gsstructuralanalysis_amd.problem import BandMatrix, ShellProblem
+    monkeypatch.setenv("KSH_BAND_PATH", "tiled")
+    pt = g.roof_problem(p=p, n=n, ny=ny, rho=1e3)
+    o = Oracle(pt)
+    print(f"[case] tiled factor roof p={p} n={n} ny={ny} nfree={o.nfree} bw={o.bw} prec={prec}", flush=True)
+    sp = ShellProblem(pt)
+    u = np.random.default_rng(5).standard_normal(o.nfree) * 1e-3
+    k = min(5, o.nfree - 1)
+    ab = _shifted(o, u, k)
+    ref = ok.BandLDLT(ab, o.nfree, o.bw)
+    assert ref.info == 0
+    A = BandMatrix(sp, precision=prec)
+    A.upload(ab)
+    A.set_inertia_guard(0.0)  # keep the FP32 factor: its count is compared too
+    A.factor()
+    neg, _ = A.inertia()
+    assert neg == ref.negatives()[0] == k
+    d, dr = A.vector_d(), ref.vectorD()
+    if prec == L.FP64:
+        # rounding of two summation orders, grown by the shifted matrix's conditioning
+        err = float(np.abs(d - dr).max() / np.abs(dr).max())
+        assert err <= 1e-9, err
+    else:
+        # FP32 pivots of a pencil this close to singular stray far from the FP64 ones (rounding
+        # of the factorization itself): compare with the one-workgroup FP32 factor instead, whose
+        # summation order differs as well, and with FP64 through the refined solution below
+        monkeypatch.setenv("KSH_BAND_PATH", "wg")
+        W = BandMatrix(sp, precision=prec)
+        W.upload(ab)
+        W.set_inertia_guard(0.0)
+        W.factor()
+        dw = W.vector_d()
+        monkeypatch.setenv("KSH_BAND_PATH", "tiled")
+        e32 = float(np.abs(d - dw).max() / np.abs(dw).max())
+        e64 = float(np.abs(dw - dr).max() / np.abs(dr).max())
+        print(f"[tiled] FP32 pivots: tiled vs one-workgroup {e32:.2e}, one-workgroup vs FP64 {e64:.2e}")
+        assert e32 <= max(2.0 * e64, 1e-5), (e32, e64)
+    b = np.random.default_rng(6).standard_normal(o.nfree)
+    xr = ref.solve(b)
+    if prec == L.FP64:
+        x = A.solve(b)
+        assert np.linalg.norm(x - xr) <= 1e-9 * np.linalg.norm(xr)
+    else:
+        x = A.solve(b, max_refine=30, tol=1e-13)
+        assert np.linalg.norm(x - xr) <= 1e-9 * np.linalg.norm(xr)
+
+
+@pytest.mark.parametrize("prec", [L.FP64, L.FP32_IR])
+def test_wide_band_takes_the_tiled_path(prec, ksh):
+    """bw = 1298: the one-workgroup factor cannot hold a panel in LDS; the library factors it over
+    the whole GPU without being told to."""
+    from gsstructuralanalysis_amd.problem import BandMatrix, ShellProblem
+    pt = g.roof_problem(p=3, n=140, ny=3, rho=1e3)
+    o = Oracle(pt)
+    print(f"[case] wide band roof p=3 n=140 ny=3 nfree={o.nfree} bw={o.bw} prec={prec}", flush=True)
+    assert o.bw > 1200
+    sp = ShellProblem(pt)
+    u = np.random.default_rng(7).standard_normal(o.nfree) * 1e-3
+    Kb = o.tangent_band(u)
+    ref = ok.BandLDLT(Kb, o.nfree, o.bw)
+    K = sp.tangent(u, precision=prec)
+    assert np.abs(K.download() - Kb).max() <= 1e-11 * np.abs(Kb).max()
+    K.factor()
+    neg, _ = K.inertia()
+    assert neg == ref.negatives()[0]
+    b = np.random.default_rng(8).standard_normal(o.nfree)
+    xr = ref.solve(b)
+    x = K.solve(b) if prec == L.FP64 else K.solve(b, max_refine=30, tol=1e-13)
+    assert np.linalg.norm(x - xr) <= 1e-9 * np.linalg.norm(xr)
