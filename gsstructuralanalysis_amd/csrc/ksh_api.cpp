@@ -147,7 +147,7 @@ void destroy_problem(Problem *pb)
 {
     if (!pb) return;
     void *ptrs[] = {pb->ctrl0, pb->free_idx, pb->free2full, pb->tabx, pb->taby, pb->wq, pb->gpgeo,
-                    pb->u_full, pb->u_free, pb->r_free, pb->Ke, pb->Fe, pb->F, pb->pl_N, pb->pl_span, pb->pl_F,
+                    pb->u_full, pb->u_free, pb->r_free, pb->Ke, pb->Hg, pb->Fe, pb->F, pb->pl_N, pb->pl_span, pb->pl_F,
                     pb->d_desc};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -343,6 +343,7 @@ int ksh_problem_create(const ksh_patch *patch, const ksh_material *mat, const ks
     if (!(P.u_free = dalloc<double>(P.nfree, rc))) return bail(rc);
     if (!(P.r_free = dalloc<double>(P.nfree, rc))) return bail(rc);
     if (!(P.Ke = dalloc<double>((size_t)P.nel * kKeTile, rc))) return bail(rc);
+    if (!(P.Hg = dalloc<double>((size_t)P.ngp * kHq, rc))) return bail(rc);
     if (!(P.Fe = dalloc<double>((size_t)P.nel * P.ne, rc))) return bail(rc);
     if (!(P.F = dalloc<double>(P.nfree, rc))) return bail(rc);
     if (!(P.pl_N = dalloc<double>(plN.size(), rc))) return bail(rc);

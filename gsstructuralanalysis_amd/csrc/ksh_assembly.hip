@@ -101,53 +101,114 @@ __global__ __launch_bounds__(256) void k_residual_elem(const InstDesc *descs, in
     }
 }
 
-// Element tangent: per component row i, H rows of component i -> Z^i = H Psi -> 3 MFMA tiles.
-template <int P, typename T>
-__global__ __launch_bounds__(256) void k_tangent_elem(const InstDesc *descs, int ninst)
+// Tangent, stage 1: one thread per Gauss point.  Derivative vectors of the deformed surface,
+// the point's state (shell::gp_state) and the 120 entries I <= J of its symmetric 15 x 15 H_g,
+// stored entry-major (Hg[q * ngp + gp]) so that a wave's stores are contiguous.  All indices
+// into the state are compile-time constants after unrolling, so it lives in registers.
+template <int P>
+__global__ __launch_bounds__(128) void k_tangent_gp(const InstDesc *descs, int ninst, int total)
 {
-    constexpr int P1 = P + 1, NB = P1 * P1, NG = P1 * P1, NKK = ((5 * NG + 3) / 4) * 4;
-    __shared__ double sX[NB][3];
-    __shared__ double sPsi[NKK][16];
-    __shared__ double sDv[NG][15];
-    __shared__ double sSt[NG][shell::kState];
-    __shared__ double sH[NG][5][15];
-    __shared__ T sZ[NKK][3][16];
-    __shared__ T sPsiT[NKK][16];
-    const InstDesc &d = descs[find_inst(ninst, blockIdx.x, [&](int m) { return (long long)descs[m].elem0; })];
-    const int e = blockIdx.x - d.elem0, ex = e % d.nx, ey = e / d.nx;
-    ElemArgs ar = elem_args(d);
-    shell::element_states<P>(ar, ex, ey, sX, sPsi, sDv, sSt, NKK);
-    for (int idx = threadIdx.x; idx < NKK * 16; idx += blockDim.x) sPsiT[idx / 16][idx % 16] = (T)sPsi[idx / 16][idx % 16];
-    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
-    T *Kout = (T *)d.Ke + (size_t)e * kKeTile;
-    for (int i = 0; i < 3; ++i) {
-        for (int idx = threadIdx.x; idx < NG * 5 * 15; idx += blockDim.x) {
-            int g = idx / 75, r = idx % 75, a = r / 15, J = r % 15;
-            sH[g][a][J] = shell::h_entry(sSt[g], 3 * a + i, J);
+    constexpr int P1 = P + 1, NB = P1 * P1, NQ = P1, NG = NQ * NQ;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= total) return;
+    const InstDesc &d = descs[find_inst(ninst, gid, [&](int m) { return (long long)descs[m].elem0 * NG; })];
+    const int gp = gid - d.elem0 * NG;
+    const int e = gp / NG, g = gp % NG, ex = e % d.nx, ey = e / d.nx, qx = g % NQ, qy = g / NQ;
+    const double *tx = d.tabx + ((size_t)(ex * NQ + qx) * 3) * P1;
+    const double *ty = d.taby + ((size_t)(ey * NQ + qy) * 3) * P1;
+    double dv[15];
+#pragma unroll
+    for (int r = 0; r < 15; ++r) dv[r] = 0.0;
+#pragma unroll
+    for (int A = 0; A < NB; ++A) {
+        const int ia = A % P1, ib = A / P1, cp = (ex + ia) + d.mx * (ey + ib);
+        const double psi[5] = {tx[P1 + ia] * ty[ib], tx[ia] * ty[P1 + ib], tx[2 * P1 + ia] * ty[ib],
+                               tx[ia] * ty[2 * P1 + ib], tx[P1 + ia] * ty[P1 + ib]};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double x = d.ctrl0[3 * cp + c] + d.u_full[3 * cp + c];
+#pragma unroll
+            for (int a = 0; a < 5; ++a) dv[3 * a + c] += psi[a] * x;
         }
-        __syncthreads();
-        for (int idx = threadIdx.x; idx < NKK * 48; idx += blockDim.x) {
-            int kk = idx / 48, col = idx % 48, k = col / 16, B = col % 16;
-            double z = 0.0;
-            if (kk < 5 * NG && B < NB) {
-                int g = kk / 5, a = kk % 5;
-                for (int b = 0; b < 5; ++b) z += sH[g][a][3 * b + k] * sPsi[5 * g + b][B];
-            }
-            sZ[kk][k][B] = (T)z;
-        }
-        __syncthreads();
-        if (wave < 3) {
-            const int k = wave;
-            typename Mfma16<T>::acc_t acc = {0, 0, 0, 0};
-            for (int st = 0; st < NKK / 4; ++st) {
-                int kk = 4 * st + lane / 16;
-                acc = Mfma16<T>::run(sPsiT[kk][lane % 16], sZ[kk][k][lane % 16], acc);
-            }
-            T *tile = Kout + (i * 3 + k) * 256;
-            for (int v = 0; v < 4; ++v) tile[Mfma16<T>::row(lane, v) * 16 + lane % 16] = acc[v];
-        }
-        __syncthreads();
     }
+    double st[shell::kState];
+    shell::gp_state(dv, d.gpgeo + (size_t)gp * kGeoStride, d.t, st);
+    const size_t stride = (size_t)d.nel * NG;
+    double *H = d.Hg + gp;
+    int q = 0;
+#pragma unroll
+    for (int I = 0; I < 15; ++I)
+#pragma unroll
+        for (int J = I; J < 15; ++J) H[(size_t)(q++) * stride] = shell::h_entry(st, I, J);
+}
+
+// index of H_g[I][J] among the stored entries I <= J (row-major upper triangle)
+__device__ __forceinline__ int hq_index(int I, int J)
+{
+    const int r = I < J ? I : J, c = I < J ? J : I;
+    return r * 15 - (r * (r - 1)) / 2 + (c - r);
+}
+
+// Tangent, stage 2: one wave per element.  K_e^{ik} = Psi^T Z^{ik} on 16 x 16 MFMA tiles
+// (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32), inner dimension kk = (g, a) over 5 NG; the
+// B operand Z^{ik}[kk][B] = sum_b H_g[(a,i),(b,k)] Psi_g[b][B] is formed in the lane that feeds
+// it, from H_g (L2) and basis values (tables, L1).  The six tiles i <= k are computed; the
+// tiles i > k are stored as their transposes.
+template <int P, typename T>
+__global__ __launch_bounds__(256) void k_tangent_tiles(const InstDesc *descs, int ninst, int total_el)
+{
+    constexpr int P1 = P + 1, NB = P1 * P1, NQ = P1, NG = NQ * NQ, NKK = ((5 * NG + 3) / 4) * 4;
+    using M = Mfma16<T>;
+    const int eg = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    if (eg >= total_el) return;
+    const InstDesc &d = descs[find_inst(ninst, eg, [&](int m) { return (long long)descs[m].elem0; })];
+    const int e = eg - d.elem0, ex = e % d.nx, ey = e / d.nx;
+    const int lane = threadIdx.x & 63, li = lane & 15, kq = lane >> 4;
+    const int ia = li % P1, ib = li / P1;
+    const bool inA = li < NB;
+    const size_t stride = (size_t)d.nel * NG;
+    const double *Hb = d.Hg + (size_t)e * NG;
+    typename M::acc_t acc[6];
+#pragma unroll
+    for (int t = 0; t < 6; ++t) acc[t] = typename M::acc_t{0, 0, 0, 0};
+    for (int s = 0; s < NKK / 4; ++s) {
+        const int kk = 4 * s + kq;
+        const bool ok = inA && kk < 5 * NG;
+        const int g = kk < 5 * NG ? kk / 5 : 0, a = kk % 5, qx = g % NQ, qy = g / NQ;
+        const double *tx = d.tabx + ((size_t)(ex * NQ + qx) * 3) * P1;
+        const double *ty = d.taby + ((size_t)(ey * NQ + qy) * 3) * P1;
+        double psi[5];
+        psi[0] = ok ? tx[P1 + ia] * ty[ib] : 0.0;
+        psi[1] = ok ? tx[ia] * ty[P1 + ib] : 0.0;
+        psi[2] = ok ? tx[2 * P1 + ia] * ty[ib] : 0.0;
+        psi[3] = ok ? tx[ia] * ty[2 * P1 + ib] : 0.0;
+        psi[4] = ok ? tx[P1 + ia] * ty[P1 + ib] : 0.0;
+        const double pa = a == 0 ? psi[0] : a == 1 ? psi[1] : a == 2 ? psi[2] : a == 3 ? psi[3] : psi[4];
+        const T aop = (T)pa;
+        const double *Hg = Hb + g;
+        int t = 0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int k = i; k < 3; ++k, ++t) {
+                double z = 0.0;
+#pragma unroll
+                for (int b = 0; b < 5; ++b) z += Hg[(size_t)hq_index(3 * a + i, 3 * b + k) * stride] * psi[b];
+                acc[t] = M::run(aop, ok ? (T)z : T(0), acc[t]);  // padded steps add exact zeros
+            }
+    }
+    T *Kout = (T *)d.Ke + (size_t)e * kKeTile;
+    int t = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = i; k < 3; ++k, ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = M::row(lane, v);
+                Kout[(i * 3 + k) * 256 + row * 16 + li] = acc[t][v];
+                if (k != i) Kout[(k * 3 + i) * 256 + li * 16 + row] = acc[t][v];
+            }
 }
 
 // consistent mass tiles: M_e^{ii}[A][B] = sum_g rho t N_A N_B wdA
@@ -412,17 +473,26 @@ int run_tangent(hipStream_t s, InstDesc *d_desc, std::vector<InstDesc> &h, bool 
         else if (f64) hipLaunchKernelGGL((k_mass_elem<3, double>), g, b, 0, s, d_desc, n);
         else hipLaunchKernelGGL((k_mass_elem<3, float>), g, b, 0, s, d_desc, n);
     } else {
+        const long long ng = (h[0].p + 1LL) * (h[0].p + 1LL), ngp = t.nel * ng;
+        if (t.nel * ng >= (1LL << 31)) return fail(KSH_OTHER_ERROR, "batch has 2^31 or more Gauss points");
+        {
+            ProfScope ps("tangent_gp", s);
+            ps.set_work((double)ngp * kHq * 8.0);  // H_g entries written (bytes)
+            if (p2) hipLaunchKernelGGL(k_tangent_gp<2>, dim3(nblk(ngp, 128)), dim3(128), 0, s, d_desc, n, (int)ngp);
+            else hipLaunchKernelGGL(k_tangent_gp<3>, dim3(nblk(ngp, 128)), dim3(128), 0, s, d_desc, n, (int)ngp);
+        }
         ProfScope ps("tangent_elem", s);
         double work = 0.0;  // B^T D B count per element and Gauss point: 72 n_e + 12 n_e^2
         for (const auto &d : h) {
-            const double ng = (d.p + 1.0) * (d.p + 1.0), ne = 3.0 * ng;
-            work += (double)d.nel * ng * (72.0 * ne + 12.0 * ne * ne);
+            const double nge = (d.p + 1.0) * (d.p + 1.0), ne = 3.0 * nge;
+            work += (double)d.nel * nge * (72.0 * ne + 12.0 * ne * ne);
         }
         ps.set_work(work);
-        if (p2 && f64) hipLaunchKernelGGL((k_tangent_elem<2, double>), g, b, 0, s, d_desc, n);
-        else if (p2) hipLaunchKernelGGL((k_tangent_elem<2, float>), g, b, 0, s, d_desc, n);
-        else if (f64) hipLaunchKernelGGL((k_tangent_elem<3, double>), g, b, 0, s, d_desc, n);
-        else hipLaunchKernelGGL((k_tangent_elem<3, float>), g, b, 0, s, d_desc, n);
+        const dim3 ge(nblk(t.nel, 4));
+        if (p2 && f64) hipLaunchKernelGGL((k_tangent_tiles<2, double>), ge, b, 0, s, d_desc, n, (int)t.nel);
+        else if (p2) hipLaunchKernelGGL((k_tangent_tiles<2, float>), ge, b, 0, s, d_desc, n, (int)t.nel);
+        else if (f64) hipLaunchKernelGGL((k_tangent_tiles<3, double>), ge, b, 0, s, d_desc, n, (int)t.nel);
+        else hipLaunchKernelGGL((k_tangent_tiles<3, float>), ge, b, 0, s, d_desc, n, (int)t.nel);
     }
     KSH_HIP(hipGetLastError());
     ProfScope ps(mass ? "mass_gather" : "band_gather", s);
@@ -466,6 +536,7 @@ InstDesc make_desc(const Problem &pb)
     d.u_full = pb.u_full;
     d.Fe = pb.Fe;
     d.Ke = pb.Ke;
+    d.Hg = pb.Hg;
     return d;
 }
 

@@ -10,6 +10,7 @@
 //   gpgeo     [nel][nq*nq][16]    undeformed geometry per Gauss point: wdA, SvK matrix C (6),
 //                                 metric A_ab (3), curvature B_ab (3)
 //   u_full    [ndof]              scratch: constructSolution of the current free vector
+//   Hg        [120][nel*ngp]     per-Gauss-point H_g of the tangent (upper triangle, FP64)
 //   Ke        [nel][9][16][16]    element tangents in (component pair, A, B) tiles (FP64 or FP32)
 //   Fe        [nel][3*nb]         element internal-force vectors
 //   F         [nfree]             external load (lambda = 1)
@@ -30,6 +31,7 @@ namespace ksh {
 constexpr int kGeoStride = 16;   // doubles per Gauss point in gpgeo
 constexpr int kTile = 16;        // element tile edge (basis functions per element padded to 16)
 constexpr int kKeTile = 9 * kTile * kTile;
+constexpr int kHq = 120;         // stored entries of a Gauss point's symmetric 15x15 H_g
 
 void set_error(const std::string &msg);
 int fail(int code, const std::string &msg);
@@ -56,6 +58,7 @@ struct InstDesc {
     const int *free_idx, *free2full;
     double *u_full, *Fe, *R;
     void *Ke, *ab;
+    double *Hg;             // per-Gauss-point H_g (upper triangle, kHq entries), entry-major
 };
 
 struct Problem;
@@ -89,6 +92,7 @@ struct Problem {
     double *u_free = nullptr;      // device staging of host free vectors
     double *r_free = nullptr;      // device staging of host results
     void *Ke = nullptr;            // FP64 or FP32 element tiles (sized for FP64)
+    double *Hg = nullptr;          // [kHq][ngp] per-Gauss-point H_g of the last tangent (FP64)
     double *Fe = nullptr;
     double *F = nullptr;
     double *pl_N = nullptr;        // [npoint][2][p+1] basis values at the load points
