@@ -10,13 +10,22 @@ TolF = 1e-8 (the metric's ||R||/||F|| bar), TolU = 1e-6, MaxIter 20, 60 steps ea
 stability (negative pivots every corrector iteration).
 
 A "step" is one converged arc-length step of one instance.  `value` = total converged steps of
-the 256 instances / wall seconds of the headline form (--form): "batched" = FP64 throughout with
-all instances' device calls merged into batched launches (default), "mixed" = the same with FP32
-band LDL^T and FP64 refinement.  Both batched forms are timed; the other one is reported beside
-the headline with its per-instance pivot counts and load factors compared against the FP64
-batch.  The plain FP64 GPU form runs one instance at a time; it is timed on one instance per mesh
-size and scaled by the size histogram of the batch.  cpu_baseline: the FP64 CPU oracle port of the same loop (reference
-re-factorisations kept) on a bounded prefix of one instance's path, on this host's cores.
+the 256 instances / wall seconds of the headline form (--form), all instances' device calls
+merged into batched launches:
+  "mixed+learned"   (default; the metric's form) FP32 band LDL^T with FP64 refinement, FP64 K, R,
+                    constraint and pivot count, and the learned arc-length predictor;
+  "mixed"           the same with the tangent predictor;
+  "batched+learned" FP64 throughout with the learned predictor;
+  "batched"         FP64 throughout with the tangent predictor.
+The learned predictor (264-256-256-65 BF16 MLP) is trained before any timing, on the GPU, on the
+FP64 paths of 256 other roofs of the same family (seeds 50261202 + i), so the timed instances are
+held out.  Every form is timed (unless --only); each is compared with the FP64 batched form
+instance by instance (pivot counts along the path, load factors).  The held-out evaluation of
+the predictor (64 roofs, seeds 30261202 + i) reports corrector iterations, factorizations and
+the fallback rate against the tangent predictor.  The plain FP64 GPU form runs one instance at
+a time; it is timed on one instance per mesh size and scaled by the size histogram of the
+batch.  cpu_baseline: the FP64 CPU oracle port of the same loop (reference re-factorisations
+kept) on a bounded prefix of one instance's path, on this host's cores.
 
 --workload c1: the single 32x32 perfect panel (the roof benchmark refined), plain vs mixed, 80 steps.
 """
@@ -35,8 +44,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "equilibrium steps/s at ‖R‖/‖F‖≤1e-8: mixed+batched+learned vs plain FP64 GPU"
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "hbm": 8.0e12}
-KERNELS = ["tangent_elem", "band_gather", "residual_elem", "vector_gather", "band_ldlt", "band_solve", "band_symv",
-           "band_ldlt_batch", "band_solve_batch", "refine_residual_batch"]
+KERNELS = ["tangent_gp", "tangent_elem", "band_gather", "residual_elem", "vector_gather", "band_ldlt", "band_solve", "band_symv",
+           "band_ldlt_batch_fp64", "band_ldlt_batch_fp32", "band_solve_batch_fp64", "band_solve_batch_fp32",
+           "refine_residual_batch", "fp64_refactor_guard", "fp64_refactor_stall", "band_ldlt_tiled", "band_solve_tiled"]
 
 
 def log(*a):
@@ -59,8 +69,11 @@ def kernel_stats(lib):
 
 
 # bound of each kernel: MFMA (work in flop) or HBM (work in bytes)
-BOUND = {"band_ldlt": "mfma", "band_ldlt_batch": "mfma", "tangent_elem": "mfma", "band_solve": "hbm",
-         "band_solve_batch": "hbm", "band_gather": "hbm", "band_symv": "hbm", "refine_residual_batch": "hbm"}
+BOUND = {"band_ldlt": "mfma", "band_ldlt_batch_fp64": "mfma", "band_ldlt_batch_fp32": "mfma", "band_ldlt_tiled": "mfma",
+         "tangent_elem": "mfma", "tangent_gp": "hbm", "band_solve": "hbm", "band_solve_batch_fp64": "hbm",
+         "band_solve_batch_fp32": "hbm", "band_solve_tiled": "hbm", "band_gather": "hbm", "band_symv": "hbm",
+         "refine_residual_batch": "hbm"}
+FP32_KERNELS = {"band_ldlt_batch_fp32"}
 
 
 def measured_traffic(kernel):
@@ -79,14 +92,14 @@ def roofline(stats, fprec_fp32, kernel=None):
     """The dominant kernel of the timed run (largest total time) against its roofline:
     achieved = algorithmic work per launch (counted by the launcher over the matrices of each
     launch) / average launch duration (HIP events on the launch stream)."""
-    cands = [k for k in stats if stats[k][0] > 0 and k in BOUND]
+    cands = [k for k in stats if stats[k][0] > 0 and stats[k][2] > 0 and k in BOUND]
     best = kernel or max(cands, key=lambda k: stats[k][1])
     cnt, ms, work = stats[best]
     per_launch = work / cnt
     avg_s = ms / cnt / 1e3
     achieved = per_launch / avg_s
     if BOUND[best] == "mfma":
-        peak = PEAK["fp32"] if (fprec_fp32 and best.startswith("band_ldlt")) else PEAK["fp64"]
+        peak = PEAK["fp32"] if best in FP32_KERNELS else PEAK["fp64"]
         return {"kernel": best, "bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": measured_traffic(best), "avg_launch_ms": ms / cnt, "launches": cnt,
                 "work_per_launch": per_launch, "work_unit": "flop"}
@@ -119,8 +132,12 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 reference timing")
-    ap.add_argument("--form", choices=["batched", "mixed"], default="batched", help="headline form")
+    ap.add_argument("--form", choices=["mixed+learned", "mixed", "batched+learned", "batched"], default="mixed+learned",
+                    help="headline form")
+    ap.add_argument("--epochs", type=int, default=150, help="training epochs of the learned predictor")
     ap.add_argument("--only", action="store_true", help="time the headline form only")
+    ap.add_argument("--max-refine", type=int, default=2, help="mixed form: refinement sweeps per solve")
+    ap.add_argument("--refine-tol", type=float, default=1e-8, help="mixed form: refinement stop ||r|| <= tol ||b||")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer 5 times (device-to-device), for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
@@ -150,8 +167,9 @@ def main():
     lib = L.load()
     kw = dict(method=0, max_iter=20, tol=1e-6, tol_f=1e-8, tol_u=1e-6, scaling=0.0, bifurcation_method=0,
               max_halvings=10)
-    forms = {"batched": dict(precision=L.FP64), "mixed": dict(precision=L.FP32_IR, max_refine=6, refine_tol=1e-11)}
-    head_kw = forms[args.form]
+    mixed = dict(precision=L.FP32_IR, max_refine=args.max_refine, refine_tol=args.refine_tol)
+    forms = {"batched": dict(precision=L.FP64), "mixed": mixed,
+             "batched+learned": dict(precision=L.FP64), "mixed+learned": dict(mixed)}
 
     if args.workload == "c1":
         steps = args.steps or 80
@@ -172,10 +190,37 @@ def main():
         workload = (f"C2 batch of {len(sps)} imperfect shallow roofs, p=3, n in {{16,24,32,40,48}}, GRF imperfection "
                     f"(l=0.1, a~U[0,0.5t]), Crisfield ds={ds}, Scaling=0, TolF=1e-8, {steps} steps each")
 
+    timed = [args.form] if args.only else [args.form] + [f for f in forms if f != args.form]
+
+    # learned predictor: trained (untimed) on the FP64 paths of other roofs of the family
+    training = None
+    net = None
+    if any(f.endswith("+learned") for f in timed):
+        from gsstructuralanalysis_amd.learned import LearnedPredictor, dataset
+        t0 = time.perf_counter()
+        ntrain = len(sps) if len(sps) > 1 else 64
+        tr_sps = [ShellProblem(g.imperfect_roof_instance(i, seed0=50261202)) for i in range(ntrain)]
+        tr, _ = run_alm_batch(tr_sps, steps, ds, features=True, **kw, **forms["batched"])
+        X, Y = dataset([r["features"][:r["steps_done"]] for r in tr])
+        t_data = time.perf_counter() - t0
+        net = LearnedPredictor(seed=1)
+        t0 = time.perf_counter()
+        losses = net.fit(X, Y, epochs=args.epochs, batch=256, lr=1e-3)
+        t_fit = time.perf_counter() - t0
+        del tr_sps
+        training = {"instances": ntrain, "seeds": "50261202 + i", "samples": int(len(X)), "epochs": args.epochs,
+                    "loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1]),
+                    "data_seconds": round(t_data, 2), "train_seconds": round(t_fit, 2)}
+        log(f"[bench] learned predictor: {len(X)} samples from {ntrain} roofs in {t_data:.1f} s; "
+            f"{args.epochs} epochs in {t_fit:.1f} s, loss {losses[0]:.3e} -> {losses[-1]:.3e}")
+        for f in forms:
+            if f.endswith("+learned"):
+                forms[f]["learned"] = net
+
     # warm-up: code objects, allocations (short separate run)
     if args.warmup > 0:
-        for f in forms.values():
-            run_alm_batch(sps[: min(8, len(sps))], args.warmup, ds, **kw, **f)
+        for f in timed:
+            run_alm_batch(sps[: min(8, len(sps))], args.warmup, ds, **kw, **forms[f])
         sps[0].run_alm(args.warmup, ds, precision=L.FP64, **kw)
 
     def run_form(f):
@@ -189,40 +234,67 @@ def main():
     # timed: headline form, per-kernel HIP events on the launching streams
     lib.ksh_profile_reset()
     lib.ksh_profile_enable(1)
-    res, t_mixed = run_form(args.form)
+    res, t_head = run_form(args.form)
     lib.ksh_profile_enable(0)
     stats = kernel_stats(lib)
     total_steps = int(sum(r["steps_done"] for r in res))
-    value = total_steps / t_mixed
-    log(f"[bench] {args.form}: {total_steps} steps in {t_mixed:.3f} s "
+    value = total_steps / t_head
+    log(f"[bench] {args.form}: {total_steps} steps in {t_head:.3f} s "
         f"-> {value:.2f} steps/s; corrector iterations {int(sum(r['iterations'].sum() for r in res))}")
     kernels = {}
     for k, (c, ms, work) in stats.items():
         if c:
             log(f"[bench]   {k:22s} launches {c:6d}  total {ms:10.2f} ms  avg {ms / c:9.4f} ms")
             if k in BOUND and work > 0:
-                r = roofline(stats, args.form == "mixed", kernel=k)
+                r = roofline(stats, args.form.startswith("mixed"), kernel=k)
                 kernels[k] = {"launches": c, "total_ms": round(ms, 2), "avg_ms": round(ms / c, 4),
                               "achieved": round(r["achieved"], 4), "unit": r["unit"], "frac": round(r["frac"], 5)}
 
-    # the other batched form, compared instance by instance with the FP64 one
-    other = None
-    if not args.only:
-        of = "mixed" if args.form == "batched" else "batched"
-        res_o, t_o = run_form(of)
-        r64, rmx = (res, res_o) if args.form == "batched" else (res_o, res)
-        neg_same = sum(int(a["steps_done"] == b["steps_done"] and np.array_equal(a["negatives"], b["negatives"]))
-                       for a, b in zip(r64, rmx))
-        dlam = max(float(np.abs(a["lambda"][:a["steps_done"]] - b["lambda"][:a["steps_done"]]).max()
-                         / max(1e-300, np.abs(a["lambda"][:a["steps_done"]]).max()))
-                   for a, b in zip(r64, rmx) if a["steps_done"] == b["steps_done"] and a["steps_done"] > 0)
-        st_o = int(sum(r["steps_done"] for r in res_o))
-        other = {"form": of, "steps_per_s": round(st_o / t_o, 3), "converged_steps": st_o,
-                 "corrector_iterations": int(sum(r["iterations"].sum() for r in res_o)),
-                 "instances_with_identical_negatives_vs_fp64_batch": neg_same, "instances": len(sps),
-                 "max_rel_lambda_diff_vs_fp64_batch": dlam}
-        log(f"[bench] {of}: {st_o} steps in {t_o:.3f} s -> {st_o / t_o:.2f} steps/s; "
-            f"identical negatives {neg_same}/{len(sps)}, max rel lambda diff {dlam:.2e}")
+    def summary(rs, secs):
+        st = int(sum(r["steps_done"] for r in rs))
+        return {"steps_per_s": round(st / secs, 3), "converged_steps": st,
+                "corrector_iterations": int(sum(r["iterations"].sum() for r in rs)),
+                "factorizations": int(sum(r["jacobians"] for r in rs)),
+                "learned_steps": int(sum(r["learned_steps"] for r in rs)),
+                "fallbacks": int(sum(r["fallbacks"] for r in rs))}
+
+    # the other forms, each compared instance by instance with the FP64 batched form
+    results = {args.form: (res, t_head)}
+    for f in timed[1:]:
+        results[f] = run_form(f)
+        log(f"[bench] {f}: {summary(*results[f])}")
+    others = {}
+    r64 = results.get("batched", (None,))[0]
+    for f, (rs, secs) in results.items():
+        o = summary(rs, secs)
+        if r64 is not None and f != "batched":
+            o["instances_with_identical_negatives_vs_fp64_batch"] = sum(
+                int(a["steps_done"] == b["steps_done"] and np.array_equal(a["negatives"], b["negatives"]))
+                for a, b in zip(r64, rs))
+            o["max_rel_lambda_diff_vs_fp64_batch"] = max(
+                float(np.abs(a["lambda"][:a["steps_done"]] - b["lambda"][:a["steps_done"]]).max()
+                      / max(1e-300, np.abs(a["lambda"][:a["steps_done"]]).max()))
+                for a, b in zip(r64, rs) if a["steps_done"] == b["steps_done"] and a["steps_done"] > 0)
+        others[f] = o
+    same_all = None
+    if r64 is not None and args.form != "batched":
+        same_all = others[args.form]["instances_with_identical_negatives_vs_fp64_batch"] == len(sps)
+
+    # held-out evaluation of the predictor: tangent vs learned on 64 roofs never trained on
+    heldout = None
+    if net is not None and len(sps) > 1:
+        prec = forms[args.form] if args.form.endswith("+learned") else forms["mixed+learned"]
+        base_kw = {k: v for k, v in prec.items() if k != "learned"}
+        ho = [ShellProblem(g.imperfect_roof_instance(i, seed0=30261202)) for i in range(64)]
+        rb, tb = run_alm_batch(ho, steps, ds, **kw, **base_kw)
+        rl, tl = run_alm_batch(ho, steps, ds, **kw, **prec)
+        sb, sl = summary(rb, tb), summary(rl, tl)
+        heldout = {"instances": 64, "seeds": "30261202 + i", "tangent": sb, "learned": sl,
+                   "fallback_rate": round(sl["fallbacks"] / max(1, sl["learned_steps"]), 4),
+                   "identical_negatives": sum(int(np.array_equal(a["negatives"], b["negatives"])) for a, b in zip(rb, rl)),
+                   "max_rel_lambda_diff": max(float(np.abs(a["lambda"] - b["lambda"]).max() / np.abs(a["lambda"]).max())
+                                              for a, b in zip(rb, rl))}
+        log(f"[bench] held-out predictor evaluation: {heldout}")
 
     # plain FP64 GPU form: one instance at a time
     plain = None
@@ -274,23 +346,28 @@ def main():
         "n_gpus": 1,
         "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * t_mixed / max(1, total_steps), 4),
+        "ms_per_step": round(1e3 * t_head / max(1, total_steps), 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.form == "batched" else "f64 (K, R, refinement, constraint) + f32 (band LDL^T)",
+        "dtype": "f64" if args.form.startswith("batched") else "f64 (K, R, refinement, constraint) + f32 (band LDL^T)",
         "data": "synthetic",
         "config": {"workload": workload,
-                   "form": {"batched": "FP64 + batched", "mixed": "mixed + batched"}[args.form]
+                   "form": {"batched": "FP64 + batched", "mixed": "mixed + batched",
+                            "batched+learned": "FP64 + batched + learned predictor",
+                            "mixed+learned": "mixed + batched + learned predictor"}[args.form]
                            + ("" if len(sps) > 1 else ", single instance"),
                    "instances": len(sps), "parallelism": "single GPU"},
         "plain_fp64_steps_per_s": round(plain, 3) if plain else None,
         "speedup_vs_plain_fp64": round(value / plain, 3) if plain else None,
         "negatives_identical_sampled": same_neg,
+        "negatives_identical_vs_fp64_batch": same_all,
         "converged_steps": total_steps,
         "corrector_iterations": int(sum(r["iterations"].sum() for r in res)),
-        "other_form": other,
-        "roofline": roofline(stats, args.form == "mixed"),
+        "forms": others,
+        "predictor_training": training,
+        "predictor_heldout": heldout,
+        "roofline": roofline(stats, args.form.startswith("mixed")),
         "kernels": kernels,
         "cpu_baseline": cpu,
     }

@@ -477,7 +477,7 @@ int run_tangent(hipStream_t s, InstDesc *d_desc, std::vector<InstDesc> &h, bool 
         if (t.nel * ng >= (1LL << 31)) return fail(KSH_OTHER_ERROR, "batch has 2^31 or more Gauss points");
         {
             ProfScope ps("tangent_gp", s);
-            ps.set_work((double)ngp * kHq * 8.0);  // H_g entries written (bytes)
+            ps.set_work((double)ngp * (kHq + kGeoStride) * 8.0);  // H_g written, undeformed geometry read (bytes)
             if (p2) hipLaunchKernelGGL(k_tangent_gp<2>, dim3(nblk(ngp, 128)), dim3(128), 0, s, d_desc, n, (int)ngp);
             else hipLaunchKernelGGL(k_tangent_gp<3>, dim3(nblk(ngp, 128)), dim3(128), 0, s, d_desc, n, (int)ngp);
         }

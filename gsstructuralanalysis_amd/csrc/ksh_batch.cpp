@@ -370,6 +370,7 @@ private:
                 Matrix &K = *grp[k]->K;
                 if (fp == KSH_FP32 && (info[k] != 0 || !(K.fstats[3] >= K.guard * K.fstats[2]))) re.push_back(&K);
             }
+            prof_count("fp64_refactor_guard", (long long)re.size());
             if ((rc = refactor64_batch(re))) return rc;
             for (size_t k = 0; k < grp.size(); ++k)
                 grp[k]->rc = grp[k]->K->info ? fail(KSH_SOLVER_ERROR, "LDL^T: zero or non-finite pivot") : KSH_SUCCESS;
@@ -516,6 +517,7 @@ private:
             std::vector<Matrix *> re;
             for (size_t i = 0; i < g.size(); ++i)
                 if (fallback[i]) re.push_back(g[i]->K);
+            prof_count("fp64_refactor_stall", (long long)re.size());
             if ((rc = refactor64_batch(re))) return rc;
             std::vector<BandDescHost> h64;
             for (size_t i = 0; i < g.size(); ++i) {

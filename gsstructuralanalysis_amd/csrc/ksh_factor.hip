@@ -456,7 +456,7 @@ int band_factor_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHo
         if (x.active && (rc = check_factor_shape(x.n, x.bw, prec, nb))) return rc;
     nb = prec == KSH_FP64 ? panel_width<double>(maxbw) : panel_width<float>(maxbw);
     KSH_HIP(hipMemcpyAsync(d_descs, dd.data(), sizeof(BandDesc) * dd.size(), hipMemcpyHostToDevice, s));
-    ProfScope ps("band_ldlt_batch", s);
+    ProfScope ps(prec == KSH_FP64 ? "band_ldlt_batch_fp64" : "band_ldlt_batch_fp32", s);
     {
         double work = 0.0;  // n (b^2 + 3b) flop per matrix
         for (const auto &x : h)

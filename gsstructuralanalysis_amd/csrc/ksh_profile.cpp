@@ -59,6 +59,13 @@ void drain(Entry &en)
 
 bool profiling() { return g_on; }
 
+void prof_count(const char *name, long long n)
+{
+    if (!g_on || n == 0) return;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_entries[name].count += n;
+}
+
 ProfScope::ProfScope(const char *name, hipStream_t s) : m_name(name), m_stream(s)
 {
     if (!g_on) return;

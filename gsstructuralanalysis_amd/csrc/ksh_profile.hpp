@@ -6,6 +6,8 @@
 namespace ksh {
 
 bool profiling();
+// adds n events to the counter `name` (reported by ksh_profile_query as launches, 0 ms)
+void prof_count(const char *name, long long n);
 
 class ProfScope {
 public:

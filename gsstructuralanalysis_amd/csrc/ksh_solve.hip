@@ -392,7 +392,7 @@ int band_solve_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHos
     }
     KSH_HIP(hipMemcpyAsync(d_descs, dd.data(), sizeof(BandDesc) * dd.size(), hipMemcpyHostToDevice, s));
     const size_t smem = solve_lds(nmax, bwmax, bytes_max, prec);
-    ProfScope ps("band_solve_batch", s);
+    ProfScope ps(prec == KSH_FP64 ? "band_solve_batch_fp64" : "band_solve_batch_fp32", s);
     {
         double work = 0.0;  // band read twice, vector in and out
         for (const auto &x : h)
