@@ -136,8 +136,12 @@ def main():
                     help="headline form")
     ap.add_argument("--epochs", type=int, default=150, help="training epochs of the learned predictor")
     ap.add_argument("--only", action="store_true", help="time the headline form only")
-    ap.add_argument("--max-refine", type=int, default=2, help="mixed form: refinement sweeps per solve")
-    ap.add_argument("--refine-tol", type=float, default=1e-8, help="mixed form: refinement stop ||r|| <= tol ||b||")
+    ap.add_argument("--also", nargs="*", default=None, metavar="FORM",
+                    help="the other forms to time (default: all); compared with the FP64 batched form")
+    ap.add_argument("--max-refine", type=int, default=3, help="mixed form: refinement sweeps per solve")
+    ap.add_argument("--refine-tol", type=float, default=1e-9, help="mixed form: refinement stop ||r|| <= tol ||b||")
+    ap.add_argument("--refine-accept", type=float, default=1e-3,
+                    help="mixed form: a solve still above accept ||b|| after the sweeps is redone with an FP64 factor")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer 5 times (device-to-device), for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
@@ -167,7 +171,8 @@ def main():
     lib = L.load()
     kw = dict(method=0, max_iter=20, tol=1e-6, tol_f=1e-8, tol_u=1e-6, scaling=0.0, bifurcation_method=0,
               max_halvings=10)
-    mixed = dict(precision=L.FP32_IR, max_refine=args.max_refine, refine_tol=args.refine_tol)
+    mixed = dict(precision=L.FP32_IR, max_refine=args.max_refine, refine_tol=args.refine_tol,
+                 refine_accept=args.refine_accept)
     forms = {"batched": dict(precision=L.FP64), "mixed": mixed,
              "batched+learned": dict(precision=L.FP64), "mixed+learned": dict(mixed)}
 
@@ -190,7 +195,10 @@ def main():
         workload = (f"C2 batch of {len(sps)} imperfect shallow roofs, p=3, n in {{16,24,32,40,48}}, GRF imperfection "
                     f"(l=0.1, a~U[0,0.5t]), Crisfield ds={ds}, Scaling=0, TolF=1e-8, {steps} steps each")
 
-    timed = [args.form] if args.only else [args.form] + [f for f in forms if f != args.form]
+    if args.only:
+        timed = [args.form]
+    else:
+        timed = [args.form] + [f for f in (args.also if args.also is not None else forms) if f != args.form]
 
     # learned predictor: trained (untimed) on the FP64 paths of other roofs of the family
     training = None

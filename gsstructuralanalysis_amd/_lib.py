@@ -41,7 +41,7 @@ class AlmOptions(C.Structure):
                 ("relaxation", C.c_double), ("quasi", C.c_int), ("quasi_iterations", C.c_int),
                 ("adaptive", C.c_int), ("adaptive_iterations", C.c_int), ("max_refine", C.c_int),
                 ("refine_tol", C.c_double), ("max_halvings", C.c_int), ("verbose", C.c_int),
-                ("inertia_guard", C.c_double), ("learned", C.c_void_p)]
+                ("inertia_guard", C.c_double), ("learned", C.c_void_p), ("refine_accept", C.c_double)]
 
 
 class AlmResult(C.Structure):
@@ -56,7 +56,7 @@ class AlmResult(C.Structure):
 class StaticOptions(C.Structure):
     _fields_ = [("precision", C.c_int), ("nsteps", C.c_int), ("dlambda", C.c_double), ("max_iter", C.c_int),
                 ("tol", C.c_double), ("max_refine", C.c_int), ("refine_tol", C.c_double), ("verbose", C.c_int),
-                ("inertia_guard", C.c_double)]
+                ("inertia_guard", C.c_double), ("refine_accept", C.c_double)]
 
 
 class StaticResult(C.Structure):

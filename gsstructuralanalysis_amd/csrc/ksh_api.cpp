@@ -440,6 +440,7 @@ int ksh_matrix_create(ksh_problem *pb, int prec, ksh_matrix **out)
     K->sprec = storage_prec(prec);
     K->fprec = factor_prec(prec);
     K->guard = pb->inertia_guard;
+    K->accept = pb->refine_accept;
     int rc = 0;
     size_t count = (size_t)K->n * K->ld, eb = elem_bytes(K->sprec), fb = elem_bytes(K->fprec);
     auto bail = [&](int code) {
@@ -643,7 +644,7 @@ int ksh_solve_dev(ksh_matrix *K, const double *b_dev, double *x_dev, int max_ref
             if ((rc = vec_norm2(s, n, r, K->red, &rn))) return rc;
             if (!(rn <= tol * bnorm) && !std::isfinite(rn)) return fail(KSH_SOLVER_ERROR, "refinement diverged");
             if (rn <= tol * bnorm) break;
-            if (it == max_refine - 1 && fp == KSH_FP32 && rn > 1e-6 * bnorm) {
+            if (it == max_refine - 1 && fp == KSH_FP32 && rn > K->accept * bnorm) {
                 // refinement stalled on the FP32 factor: solve with an FP64 factor instead
                 if ((rc = inertia_fp64(*K, nullptr, nullptr))) return rc;
                 KSH_HIP(hipMemcpyAsync(x_dev, bcopy, sizeof(double) * n, hipMemcpyDeviceToDevice, s));

@@ -469,7 +469,8 @@ private:
         if (max_refine > 0 && (rc = m_part.ensure(ptot * sizeof(double)))) return rc;
         // Refinement, instance by instance the same decisions as the single-problem solve
         // (ksh_solve_dev): stop at ||r|| <= tol ||b||; on the last sweep of an FP32 factor whose
-        // residual is still above 1e-6 ||b||, replace the factor by an FP64 one and solve again.
+        // residual is still above accept ||b|| (K.accept, default 1e-6), replace the factor by an
+        // FP64 one and solve again.
         std::vector<int> efp(g.size()), fallback(g.size(), 0), err(g.size(), 0);
         for (size_t i = 0; i < g.size(); ++i) {
             efp[i] = g[i]->K->use64 ? KSH_FP64 : g[i]->K->fprec;
@@ -497,7 +498,7 @@ private:
                     active[i] = 0;
                 } else if (conv) {
                     active[i] = 0;
-                } else if (it == g[i]->max_refine - 1 && efp[i] == KSH_FP32 && rn > 1e-6 * bn) {
+                } else if (it == g[i]->max_refine - 1 && efp[i] == KSH_FP32 && rn > g[i]->K->accept * bn) {
                     fallback[i] = 1;
                     active[i] = 0;
                 } else {

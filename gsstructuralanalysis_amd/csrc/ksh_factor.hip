@@ -387,7 +387,7 @@ int check_factor_shape(int n, int bw, int prec, int &nb)
 }
 
 // single bands of at least this many multiply-adds (n bw^2) are factored over the whole GPU
-constexpr double kTiledFactorWork = 1e30;
+constexpr double kTiledFactorWork = 1e9;
 
 }  // namespace
 

@@ -79,6 +79,7 @@ struct Problem {
     int device = 0;
     hipStream_t stream = nullptr;
     double inertia_guard = 1e-6;   // guard given to the matrices created for this problem
+    double refine_accept = 1e-6;   // FP32-factor solves above this relative residual go FP64
     std::vector<int> h_free2full;  // host copy of free2full
     std::shared_ptr<void> modal;   // learned-predictor modal basis (built on first use)
 
@@ -118,6 +119,7 @@ struct Matrix {
     uint64_t factored_version = ~0ull;
     int info = 0;                  // 0 or 1-based index of a zero / non-finite pivot
     double guard = 1e-6;
+    double accept = 1e-6;          // refined FP32 solves above accept ||b|| are redone in FP64
     double fstats[4] = {0, 0, 0, 0};  // pivot statistics of the current factor (negatives, min d, max|d|, min|d|)
     bool has_stats = false;
     // device workspace for solves and inertia

@@ -40,6 +40,7 @@ int alm_driver(ksh_problem *pb, gsExec *exec, const ksh_alm_options *o, ksh_alm_
         gsSparseSolver<real_t>::settings().max_refine = o->max_refine;
         gsSparseSolver<real_t>::settings().refine_tol = o->refine_tol;
         if (o->inertia_guard > 0) pb->inertia_guard = o->inertia_guard;
+        if (o->refine_accept > 0) pb->refine_accept = o->refine_accept;
         gsShellOps ops(pb, o->precision, exec);
         const index_t n = ops.numDofs();
         gsVector<real_t> Force = ops.force();
@@ -166,6 +167,7 @@ extern "C" int ksh_run_static(ksh_problem *pb, const ksh_static_options *o, ksh_
         gsSparseSolver<real_t>::settings().max_refine = o->max_refine;
         gsSparseSolver<real_t>::settings().refine_tol = o->refine_tol;
         if (o->inertia_guard > 0) pb->inertia_guard = o->inertia_guard;
+        if (o->refine_accept > 0) pb->refine_accept = o->refine_accept;
         gsShellOps ops(pb, o->precision);
         const index_t n = ops.numDofs();
         double t0 = now_s();

@@ -29,13 +29,13 @@ def _f64(a):
 def alm_options(nsteps, length, method=0, precision=L.FP64, max_iter=20, tol=1e-6, tol_u=1e-6, tol_f=1e-3,
                 scaling=-1.0, angle_method=0, bifurcation_method=0, relaxation=1.0, quasi=0, quasi_iterations=-1,
                 adaptive=0, adaptive_iterations=10, max_refine=0, refine_tol=1e-14, max_halvings=10, verbose=0,
-                inertia_guard=0.0, learned=None):
+                inertia_guard=0.0, learned=None, refine_accept=0.0):
     """ksh_alm_options with the reference's option names in snake case (gsALMBase.hpp:22-54);
     `learned` is a LearnedPredictor (Crisfield only)."""
     return L.AlmOptions(method, precision, nsteps, length, max_iter, tol, tol_u, tol_f, scaling, angle_method,
                         bifurcation_method, relaxation, quasi, quasi_iterations, adaptive, adaptive_iterations,
                         max_refine, refine_tol, max_halvings, verbose, inertia_guard,
-                        learned.handle if learned is not None else None)
+                        learned.handle if learned is not None else None, refine_accept)
 
 
 def _alm_result(nsteps, nfree, keep_U, probe_dof, features=False):
@@ -259,8 +259,9 @@ class ShellProblem:
         return self._run_eig(L.load().ksh_run_modal, number, keep_vectors, kw)
 
     def run_static(self, nsteps, dlambda, precision=L.FP64, max_iter=25, tol=1e-10, max_refine=0,
-                   refine_tol=1e-14, verbose=0, keep_U=False, inertia_guard=0.0):
-        o = L.StaticOptions(precision, nsteps, dlambda, max_iter, tol, max_refine, refine_tol, verbose, inertia_guard)
+                   refine_tol=1e-14, verbose=0, keep_U=False, inertia_guard=0.0, refine_accept=0.0):
+        o = L.StaticOptions(precision, nsteps, dlambda, max_iter, tol, max_refine, refine_tol, verbose, inertia_guard,
+                            refine_accept)
         out = {k: np.zeros(nsteps) for k in ("lambda", "unorm", "residual")}
         out["iterations"] = np.zeros(nsteps, dtype=np.int32)
         out["negatives"] = np.zeros(nsteps, dtype=np.int64)

@@ -181,6 +181,9 @@ typedef struct {
     struct ksh_mlp *learned; /* Crisfield only: learned predictor network (NULL: tangent   */
                              /* predictor); a step that fails from the learned proposal   */
                              /* is redone from the tangent predictor                      */
+    double refine_accept;    /* FP32 factors: a solve whose residual is still above        */
+                             /* refine_accept ||b|| after max_refine sweeps is redone      */
+                             /* with an FP64 factor (0: the library default, 1e-6)         */
 } ksh_alm_options;
 
 typedef struct {
@@ -226,6 +229,7 @@ typedef struct {
     double refine_tol;
     int verbose;
     double inertia_guard;    /* as in ksh_alm_options                                      */
+    double refine_accept;    /* as in ksh_alm_options                                      */
 } ksh_static_options;
 
 typedef struct {
