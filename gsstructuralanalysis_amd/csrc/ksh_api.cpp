@@ -157,7 +157,7 @@ void destroy_problem(Problem *pb)
 void destroy_matrix(Matrix *K)
 {
     if (!K) return;
-    void *ptrs[] = {K->ab, K->fac, K->fac64, K->u_at, K->work, K->flag, K->red};
+    void *ptrs[] = {K->ab, K->fac, K->fac64, K->u_at, K->work, K->flag, K->red, K->desc};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -453,6 +453,7 @@ int ksh_matrix_create(ksh_problem *pb, int prec, ksh_matrix **out)
     if (!(K->work = dalloc<double>((size_t)5 * K->n, rc))) return bail(rc);
     if (!(K->flag = dalloc<int>(4, rc))) return bail(rc);
     if (!(K->red = dalloc<double>(8, rc))) return bail(rc);
+    if (!(K->desc = dalloc<unsigned char>(kBandDescBytes, rc))) return bail(rc);
     if (K->sprec == KSH_FP32 && !(K->u_at = dalloc<double>(K->n, rc))) return bail(rc);
     // zeroed before return: the matrix may next be written on another stream (a batch engine's),
     // which must not race with this clear

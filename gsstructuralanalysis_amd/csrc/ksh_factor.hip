@@ -386,20 +386,17 @@ int check_factor_shape(int n, int bw, int prec, int &nb)
     return KSH_SUCCESS;
 }
 
-// single bands of at least this many multiply-adds (n bw^2) are factored over the whole GPU
-constexpr double kTiledFactorWork = 1e9;
-
 }  // namespace
 
-// KSH_BAND_PATH=tiled / wg forces the multi-workgroup / one-workgroup path where both apply
+// Default: the multi-workgroup factor (ksh_factor_tiled.hip) for single bands and batches alike.
+// KSH_BAND_PATH=wg selects this file's one-workgroup-per-band factor where the bands' panels fit
+// LDS (it computes the same factorization with another summation order).
 bool band_use_tiled_factor(int n, int bw, int prec)
 {
+    (void)n;
     const char *e = std::getenv("KSH_BAND_PATH");
-    if (e && std::strcmp(e, "tiled") == 0) return true;
-    const int nb = prec == KSH_FP64 ? panel_width<double>(bw) : panel_width<float>(bw);
-    if (nb == 0) return true;
-    if (e && std::strcmp(e, "wg") == 0) return false;
-    return (double)n * bw * bw >= kTiledFactorWork;
+    if (!e || std::strcmp(e, "wg") != 0) return true;
+    return (prec == KSH_FP64 ? panel_width<double>(bw) : panel_width<float>(bw)) == 0;
 }
 
 int band_factor(Matrix &K, void *fac, int prec, double *stats_out)
@@ -420,7 +417,7 @@ int band_factor(Matrix &K, void *fac, int prec, double *stats_out)
     if (tiled) {
         ProfScope ps("band_ldlt_tiled", pb.stream);
         ps.set_work((double)K.n * ((double)K.bw * K.bw + 3.0 * K.bw));
-        rc = band_factor_tiled(pb.stream, fac, prec, K.n, K.bw, K.flag, dstats);
+        rc = band_factor_tiled_batch(pb.stream, K.desc, {BandDescHost{fac, K.n, K.bw, K.flag, dstats, nullptr, 1}}, prec);
     } else {
         ProfScope ps("band_ldlt", pb.stream);
         ps.set_work((double)K.n * ((double)K.bw * K.bw + 3.0 * K.bw));
@@ -451,11 +448,6 @@ int band_factor_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHo
         maxbw = std::max(maxbw, h[i].bw);
         maxn = std::max(maxn, h[i].n);
     }
-    int nb, rc;
-    for (const auto &x : h)
-        if (x.active && (rc = check_factor_shape(x.n, x.bw, prec, nb))) return rc;
-    nb = prec == KSH_FP64 ? panel_width<double>(maxbw) : panel_width<float>(maxbw);
-    KSH_HIP(hipMemcpyAsync(d_descs, dd.data(), sizeof(BandDesc) * dd.size(), hipMemcpyHostToDevice, s));
     ProfScope ps(prec == KSH_FP64 ? "band_ldlt_batch_fp64" : "band_ldlt_batch_fp32", s);
     {
         double work = 0.0;  // n (b^2 + 3b) flop per matrix
@@ -463,6 +455,12 @@ int band_factor_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHo
             if (x.active) work += (double)x.n * ((double)x.bw * x.bw + 3.0 * x.bw);
         ps.set_work(work);
     }
+    if (band_use_tiled_factor(maxn, maxbw, prec)) return band_factor_tiled_batch(s, d_descs, h, prec);
+    int nb, rc;
+    for (const auto &x : h)
+        if (x.active && (rc = check_factor_shape(x.n, x.bw, prec, nb))) return rc;
+    nb = prec == KSH_FP64 ? panel_width<double>(maxbw) : panel_width<float>(maxbw);
+    KSH_HIP(hipMemcpyAsync(d_descs, dd.data(), sizeof(BandDesc) * dd.size(), hipMemcpyHostToDevice, s));
     return prec == KSH_FP64 ? launch_factor<double>(s, nb, (unsigned)dd.size(), (const BandDesc *)d_descs, nullptr, 0, maxbw,
                                                     nullptr, nullptr)
                             : launch_factor<float>(s, nb, (unsigned)dd.size(), (const BandDesc *)d_descs, nullptr, 0, maxbw,

@@ -1,25 +1,34 @@
-// ksh_factor_tiled.hip -- banded LDL^T factorization spread over the whole GPU, for one large
-// band (wide bands whose panel does not fit one workgroup's LDS, or long ones where a single
-// workgroup would leave the other CUs idle).
+// ksh_factor_tiled.hip -- banded LDL^T factorization of a batch of bands spread over the whole
+// GPU: every panel step is a pair of launches covering all bands still being factored, so a
+// single large band (the 128 x 128 plate: n 49k, bw 1166) and a batch of uneven ones (the
+// imperfection sweep: bw 182..470) both keep all CUs busy, and no panel has to fit LDS.
 //
-// Same algorithm and layout as the one-workgroup factor (ksh_factor.hip): lower band, column
-// major, fac[j*ld + (i-j)], ld = bw+1, right-looking in panels of NB = 32 columns, no pivoting
-// (SimplicialLDLT::compute on the eigen, Newton and arc-length paths,
-// src/gsEigenSolvers/gsBucklingSolver.h:131-133, src/gsALMSolvers/gsALMBase.hpp:186-211).
-// Each panel step is two launches on the stream:
-//   1. k_band_panel (one workgroup): wave 0 factors the NB x NB diagonal block in registers
-//      (lane = row, v_readlane broadcasts); every thread then solves one row below it against
-//      L_dd D (row-wise TRSM) straight from HBM, so no panel has to fit LDS;
-//   2. k_band_trailing (a workgroup of 4 waves per 4 blocks): the lower triangle of the
+// Replaces SimplicialLDLT::compute on the arc-length, Newton and eigen paths
+// (src/gsALMSolvers/gsALMBase.hpp:186-211, src/gsStaticSolvers/gsStaticNewton.hpp:215-240,
+// src/gsEigenSolvers/gsBucklingSolver.h:131-133).  No pivoting, as there; by Sylvester's law of
+// inertia the number of negative pivots does not depend on the ordering, so the band ordering
+// reproduces the count gsALMBase::_computeStability takes (src/gsALMSolvers/gsALMHelper.h:30-42).
+//
+// Layout: lower band, column major, fac[j*ld + (i-j)], ld = bw+1, factored in place into unit L
+// and D.  Right-looking in panels of NB = 32 columns.  Panel step at c0:
+//   1. k_band_panel (one workgroup of 512 threads per band): the NB x NB diagonal block is
+//      staged in LDS and wave 0 factors it (lane = row; column jj is shared through LDS) and
+//      writes it back; then each thread solves rows below it against L_dd D (row-wise TRSM),
+//      read and written straight from the band.  (One workgroup per band: a second one would
+//      read the diagonal block while the first writes it back.)
+//   2. k_band_trailing (per band a workgroup of 4 waves per 4 blocks): the lower triangle of the
 //      bw x bw trailing window is cut into 32 x 32 blocks, one per wave, each a 2 x 2 set of
-//      16 x 16 MFMA tiles (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32); the panel operands
-//      -d_k L[j][k] and L[i][k] are read from L2 (the panel was just written), the block is read,
-//      updated and written once.
-// Work per matrix: about n bw^2 multiply-adds; HBM traffic about 2 n bw^2 / NB entries.
+//      16 x 16 MFMA tiles (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32) over the NB panel
+//      columns; panel operands -d_k L[j][k] and L[i][k] come from L2, the block is read, updated
+//      and written once.
+// Bands are sorted by order, largest first, so a step's grids cover exactly the bands that are
+// still active.  Work per band: about n bw^2 multiply-adds; HBM traffic about 2 n bw^2 / NB
+// entries (the trailing window of a step, read and written).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <string>
+#include <vector>
 
 #include "ksh_devutil.hpp"
 #include "ksh_internal.hpp"
@@ -30,21 +39,25 @@ namespace ksh {
 
 namespace {
 
-constexpr int kNB = 32;       // panel width
-constexpr int kPT = 256;      // panel kernel threads
-constexpr int kUW = 4;        // waves per trailing-update workgroup
+constexpr int kNB = 32;   // panel width
+constexpr int kPT = 512;  // panel kernel threads (TRSM rows per pass)
+constexpr int kUW = 4;    // waves per trailing-update workgroup
 
 template <typename T>
-__global__ __launch_bounds__(kPT) void k_band_panel(T *fac, int n, int bw, int c0, int *info)
+__global__ __launch_bounds__(kPT) void k_band_panel(const BandDesc *descs, int c0)
 {
     __shared__ T blk[kNB * kNB];  // blk[kk*NB + r] = A / L[c0+r][c0+kk] for r >= kk, zero above
     __shared__ T dinv[kNB];
     __shared__ T colv[2][kNB];    // column of the current elimination step (double-buffered)
+    const BandDesc &d = descs[blockIdx.x];
+    const int n = d.n, bw = d.bw;
+    if (c0 >= n) return;
+    const int w = min(kNB, n - c0), nr = min(n - c0, w + bw);
+    T *fac = (T *)d.fac;
     const unsigned ldu = (unsigned)bw + 1;
     constexpr unsigned es = sizeof(T);
     const __amdgpu_buffer_rsrc_t rs = band_rsrc(fac, (unsigned)((size_t)n * ldu * es));
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int w = min(kNB, n - c0), nr = min(n - c0, w + bw);
     constexpr int kQ = kNB * kNB / kPT;
     // stage the diagonal block (unit pivots pad the last panel)
 #pragma unroll
@@ -56,9 +69,8 @@ __global__ __launch_bounds__(kPT) void k_band_panel(T *fac, int n, int bw, int c
     }
     __syncthreads();
     if (wave == 0) {
-        // lane r < NB owns row c0+r; lanes >= NB shadow row r-NB.  Step jj shares column jj
-        // through LDS (every lane reads the same entries: broadcasts).  Entries above the
-        // diagonal take updates too; they are never read.
+        // lane r owns row c0+r.  Step jj shares column jj through LDS (every lane reads the same
+        // entries: broadcasts).  Entries above the diagonal take updates too; never read.
         const int r = lane & (kNB - 1);
         T a[kNB];
 #pragma unroll
@@ -67,7 +79,7 @@ __global__ __launch_bounds__(kPT) void k_band_panel(T *fac, int n, int bw, int c
 #pragma unroll
         for (int jj = 0; jj < kNB; ++jj) {
             T *col = colv[jj & 1];
-            if (lane < kNB) col[r] = a[jj];
+            col[r] = a[jj];
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
             const T djj = col[jj];
@@ -79,12 +91,10 @@ __global__ __launch_bounds__(kPT) void k_band_panel(T *fac, int n, int bw, int c
             __builtin_amdgcn_sched_barrier(0);  // keeps the steps' LDS reads from being hoisted together
         }
         // first bad pivot of the factorization: the first panel clears the flag
-        if (lane == 0 && (c0 == 0 || (bad != 0 && *info == 0))) *info = bad;
-        if (lane < kNB) {
-            dinv[r] = T(1) / a[r];
+        if (lane == 0 && d.info && (c0 == 0 || (bad != 0 && *d.info == 0))) *d.info = bad;
+        dinv[r] = T(1) / a[r];
 #pragma unroll
-            for (int kk = 0; kk < kNB; ++kk) blk[kk * kNB + r] = kk <= r ? a[kk] : T(0);
-        }
+        for (int kk = 0; kk < kNB; ++kk) blk[kk * kNB + r] = kk <= r ? a[kk] : T(0);
     }
     __syncthreads();
     // write the factored block back (L below the diagonal, D on it)
@@ -94,7 +104,7 @@ __global__ __launch_bounds__(kPT) void k_band_panel(T *fac, int n, int bw, int c
         const bool ok = r >= kk && r < w && r - kk <= bw;
         store_if<T>(ok, rs, ((unsigned)(c0 + kk) * ldu + (unsigned)(r - kk)) * es, blk[idx]);
     }
-    // TRSM: rows r = w .. nr-1 of the panel, L[r][0:NB] = A[r][0:NB] (D L_dd^T)^-1
+    // TRSM: row r of the panel, L[r][0:NB] = A[r][0:NB] (D L_dd^T)^-1
     for (int r = w + tid; r < nr; r += kPT) {
         T a[kNB];
 #pragma unroll
@@ -130,33 +140,26 @@ __device__ __forceinline__ void tri_decode(int p, int &bi, int &bj)
 
 // A[i][j] -= sum_k L[i][k] d_k L[j][k] over the window rows/columns [w, nr) of the frame at c0
 template <typename T>
-__global__ __launch_bounds__(64 * kUW) void k_band_trailing(T *fac, int n, int bw, int c0, int w, int nr, int nblock)
+__global__ __launch_bounds__(64 * kUW) void k_band_trailing(const BandDesc *descs, int c0, int wg_per_band)
 {
     using M = Mfma16<T>;
     using acc_t = typename M::acc_t;
+    const BandDesc &d = descs[blockIdx.x / wg_per_band];
+    const int n = d.n, bw = d.bw;
+    if (c0 >= n) return;
+    const int w = min(kNB, n - c0), nr = min(n - c0, w + bw);
+    const int nbk = (nr - w + 31) / 32, nblock = nbk * (nbk + 1) / 2;
+    const int p = __builtin_amdgcn_readfirstlane((int)((blockIdx.x % wg_per_band) * kUW + (threadIdx.x >> 6)));
+    if (p >= nblock) return;
+    T *fac = (T *)d.fac;
     const unsigned ldu = (unsigned)bw + 1;
     constexpr unsigned es = sizeof(T);
     const __amdgpu_buffer_rsrc_t rs = band_rsrc(fac, (unsigned)((size_t)n * ldu * es));
     const int lane = threadIdx.x & 63;
-    const int p = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kUW + (threadIdx.x >> 6)));
-    if (p >= nblock) return;
     int bi, bj;
     tri_decode(p, bi, bj);
     const int li = lane & 15, lk = lane >> 4;
     const bool diag = bi == bj;
-    // panel operands: rows rj (block column bj) and ri (block row bi), k = 4 s + lk
-    const int ri0 = w + 32 * bi + li, rj0 = w + 32 * bj + li;
-    T A0[kNB / 4], A1[kNB / 4], B0[kNB / 4], B1[kNB / 4];
-#pragma unroll
-    for (int s = 0; s < kNB / 4; ++s) {
-        const int k = 4 * s + lk;
-        const unsigned col = ((unsigned)c0 * ldu + (unsigned)k * (ldu - 1)) * es;  // + row * es
-        const T nd = -load_if<T>(k < w, rs, (unsigned)(c0 + k) * ldu * es);
-        A0[s] = nd * load_if<T>(k < w && rj0 < nr && rj0 - k <= bw, rs, col + (unsigned)rj0 * es);
-        A1[s] = nd * load_if<T>(k < w && rj0 + 16 < nr && rj0 + 16 - k <= bw, rs, col + (unsigned)(rj0 + 16) * es);
-        B0[s] = load_if<T>(k < w && ri0 < nr && ri0 - k <= bw, rs, col + (unsigned)ri0 * es);
-        B1[s] = load_if<T>(k < w && ri0 + 16 < nr && ri0 + 16 - k <= bw, rs, col + (unsigned)(ri0 + 16) * es);
-    }
     // accumulator rows = j (A's column), columns = i (A's row): lanes run along a column
     acc_t cur[2][2];
     unsigned off[2][2][4];
@@ -171,12 +174,29 @@ __global__ __launch_bounds__(64 * kUW) void k_band_trailing(T *fac, int n, int b
                 off[a][b][v] = ok ? ((unsigned)c0 * ldu + (unsigned)j * (ldu - 1) + (unsigned)i) * es : kMaskedOff;
                 cur[a][b][v] = load_vs<T>(rs, off[a][b][v], 0);
             }
+    // panel operands: rows rj (block column bj) and ri (block row bi), k = 4 s + lk, in chunks
+    const int ri0 = w + 32 * bi + li, rj0 = w + 32 * bj + li;
+    constexpr int kCh = 8;
 #pragma unroll
-    for (int s = 0; s < kNB / 4; ++s) {
-        cur[0][0] = M::run(A0[s], B0[s], cur[0][0]);
-        cur[1][0] = M::run(A0[s], B1[s], cur[1][0]);
-        cur[1][1] = M::run(A1[s], B1[s], cur[1][1]);
-        if (!diag) cur[0][1] = M::run(A1[s], B0[s], cur[0][1]);
+    for (int c = 0; c < kNB / 4 / kCh; ++c) {
+        T A0[kCh], A1[kCh], B0[kCh], B1[kCh];
+#pragma unroll
+        for (int u = 0; u < kCh; ++u) {
+            const int k = 4 * (kCh * c + u) + lk;
+            const unsigned col = ((unsigned)c0 * ldu + (unsigned)k * (ldu - 1)) * es;  // + row * es
+            const T nd = -load_if<T>(k < w, rs, (unsigned)(c0 + k) * ldu * es);
+            A0[u] = nd * load_if<T>(k < w && rj0 < nr && rj0 - k <= bw, rs, col + (unsigned)rj0 * es);
+            A1[u] = nd * load_if<T>(k < w && rj0 + 16 < nr && rj0 + 16 - k <= bw, rs, col + (unsigned)(rj0 + 16) * es);
+            B0[u] = load_if<T>(k < w && ri0 < nr && ri0 - k <= bw, rs, col + (unsigned)ri0 * es);
+            B1[u] = load_if<T>(k < w && ri0 + 16 < nr && ri0 + 16 - k <= bw, rs, col + (unsigned)(ri0 + 16) * es);
+        }
+#pragma unroll
+        for (int u = 0; u < kCh; ++u) {
+            cur[0][0] = M::run(A0[u], B0[u], cur[0][0]);
+            cur[1][0] = M::run(A0[u], B1[u], cur[1][0]);
+            cur[1][1] = M::run(A1[u], B1[u], cur[1][1]);
+            if (!diag) cur[0][1] = M::run(A1[u], B0[u], cur[0][1]);
+        }
     }
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -186,14 +206,17 @@ __global__ __launch_bounds__(64 * kUW) void k_band_trailing(T *fac, int n, int b
             for (int v = 0; v < 4; ++v) store_vs<T>(rs, off[a][b][v], 0, cur[a][b][v]);
 }
 
-// negatives, min d, max |d|, min |d| of the factored diagonal (one workgroup of 1024 threads)
+// negatives, min d, max |d|, min |d| of each factored diagonal (one workgroup of 1024 threads per band)
 template <typename T>
-__global__ __launch_bounds__(1024) void k_pivot_stats(const T *fac, int n, int bw, double *stats)
+__global__ __launch_bounds__(1024) void k_pivot_stats(const BandDesc *descs)
 {
     __shared__ double part[16][4];
-    const size_t ld = (size_t)bw + 1;
+    const BandDesc &dd = descs[blockIdx.x];
+    if (!dd.stats) return;
+    const T *fac = (const T *)dd.fac;
+    const size_t ld = (size_t)dd.bw + 1;
     double c = 0.0, mn = INFINITY, mxa = 0.0, mna = INFINITY;
-    for (int i = threadIdx.x; i < n; i += 1024) {
+    for (int i = threadIdx.x; i < dd.n; i += 1024) {
         const double d = (double)fac[(size_t)i * ld];
         c += d < 0.0 ? 1.0 : 0.0;
         mn = fmin(mn, d);
@@ -221,40 +244,55 @@ __global__ __launch_bounds__(1024) void k_pivot_stats(const T *fac, int n, int b
             mxa = fmax(mxa, part[q][2]);
             mna = fmin(mna, part[q][3]);
         }
-        stats[0] = c;
-        stats[1] = mn;
-        stats[2] = mxa;
-        stats[3] = mna;
+        dd.stats[0] = c;
+        dd.stats[1] = mn;
+        dd.stats[2] = mxa;
+        dd.stats[3] = mna;
     }
 }
 
 template <typename T>
-int run_tiled(hipStream_t s, T *fac, int n, int bw, int *info, double *stats)
+void run_tiled(hipStream_t s, const BandDesc *d_descs, const std::vector<int> &ns, int maxbw)
 {
-    for (int c0 = 0; c0 < n; c0 += kNB) {
-        const int w = std::min(kNB, n - c0), nr = std::min(n - c0, w + bw);
-        hipLaunchKernelGGL(k_band_panel<T>, dim3(1), dim3(kPT), 0, s, fac, n, bw, c0, info);
-        if (nr > w) {
-            const int nbk = (nr - w + 31) / 32, nblock = nbk * (nbk + 1) / 2;
-            hipLaunchKernelGGL(k_band_trailing<T>, dim3((unsigned)((nblock + kUW - 1) / kUW)), dim3(64 * kUW), 0, s, fac, n, bw,
-                               c0, w, nr, nblock);
-        }
+    // ns: orders of the bands, descending (the device array is in the same order)
+    const int nbands = (int)ns.size();
+    const int nbk = (maxbw + 31) / 32, wg_trail = std::max(1, (nbk * (nbk + 1) / 2 + kUW - 1) / kUW);
+    int active = nbands;
+    for (int c0 = 0; c0 < ns[0]; c0 += kNB) {
+        while (active > 0 && ns[active - 1] <= c0) --active;
+        hipLaunchKernelGGL(k_band_panel<T>, dim3((unsigned)active), dim3(kPT), 0, s, d_descs, c0);
+        hipLaunchKernelGGL(k_band_trailing<T>, dim3((unsigned)(active * wg_trail)), dim3(64 * kUW), 0, s, d_descs, c0,
+                           wg_trail);
     }
-    if (stats) hipLaunchKernelGGL(k_pivot_stats<T>, dim3(1), dim3(1024), 0, s, (const T *)fac, n, bw, stats);
-    KSH_HIP(hipGetLastError());
-    return KSH_SUCCESS;
+    hipLaunchKernelGGL(k_pivot_stats<T>, dim3((unsigned)nbands), dim3(1024), 0, s, d_descs);
 }
 
 }  // namespace
 
-int band_factor_tiled(hipStream_t s, void *fac, int prec, int n, int bw, int *info, double *stats)
+int band_factor_tiled_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHost> &h, int prec)
 {
     const size_t es = prec == KSH_FP64 ? sizeof(double) : sizeof(float);
-    if ((size_t)n * (size_t)(bw + 1) * es >= 0x80000000ull)
-        return fail(KSH_SOLVER_ERROR, "band of 2 GiB or more: beyond the band kernels' 32-bit offsets");
-    if (n < 1) return fail(KSH_SOLVER_ERROR, "empty band");
-    return prec == KSH_FP64 ? run_tiled<double>(s, (double *)fac, n, bw, info, stats)
-                            : run_tiled<float>(s, (float *)fac, n, bw, info, stats);
+    std::vector<BandDesc> dd;
+    for (const auto &x : h) {
+        if (!x.active) continue;
+        if (x.n < 1) return fail(KSH_SOLVER_ERROR, "empty band");
+        if ((size_t)x.n * (size_t)(x.bw + 1) * es >= 0x80000000ull)
+            return fail(KSH_SOLVER_ERROR, "band of 2 GiB or more: beyond the band kernels' 32-bit offsets");
+        dd.push_back(BandDesc{x.fac, x.n, x.bw, x.info, x.stats, x.x, 1, 0});
+    }
+    if (dd.empty()) return KSH_SUCCESS;
+    std::stable_sort(dd.begin(), dd.end(), [](const BandDesc &a, const BandDesc &b) { return a.n > b.n; });
+    std::vector<int> ns(dd.size());
+    int maxbw = 0;
+    for (size_t i = 0; i < dd.size(); ++i) {
+        ns[i] = dd[i].n;
+        maxbw = std::max(maxbw, dd[i].bw);
+    }
+    KSH_HIP(hipMemcpyAsync(d_descs, dd.data(), sizeof(BandDesc) * dd.size(), hipMemcpyHostToDevice, s));
+    if (prec == KSH_FP64) run_tiled<double>(s, (const BandDesc *)d_descs, ns, maxbw);
+    else run_tiled<float>(s, (const BandDesc *)d_descs, ns, maxbw);
+    KSH_HIP(hipGetLastError());
+    return KSH_SUCCESS;
 }
 
 }  // namespace ksh

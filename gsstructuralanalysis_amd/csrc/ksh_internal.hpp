@@ -125,6 +125,7 @@ struct Matrix {
     // device workspace for solves and inertia
     double *work = nullptr;        // 5*n doubles (the last n: tiled-solve workspace)
     int *flag = nullptr;           // small device int buffer
+    void *desc = nullptr;          // device descriptor of this band for batched kernels (one BandDesc)
     double *red = nullptr;         // reduction scratch
 };
 
@@ -159,6 +160,9 @@ struct BandDesc {
     int pad_;
 };
 int band_factor_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHost> &h, int prec);
+// factor of a batch of bands over the whole GPU (ksh_factor_tiled.hip): panel steps of two
+// launches covering every band still active; d_descs holds h.size() device BandDescs
+int band_factor_tiled_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHost> &h, int prec);
 // batched refinement: r = b - K x with norms (||r||^2, ||b||^2), then x += r on active matrices
 struct RefDesc {
     const void *ab;
@@ -177,9 +181,7 @@ int ensure_fac64(Matrix &K);
 int band_solve_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHost> &h, int prec);
 constexpr size_t kBandDescBytes = 48;
 int band_solve(Matrix &K, const void *fac, int prec, double *x_dev);
-// one large band over the whole GPU (ksh_factor_tiled.hip, ksh_solve_tiled.hip): panel steps of
-// two launches; stats (optional) as band_factor's; the solve uses z (n doubles) as workspace
-int band_factor_tiled(hipStream_t s, void *fac, int prec, int n, int bw, int *info, double *stats);
+// solve of one large band over many workgroups (ksh_solve_tiled.hip); z: n doubles of workspace
 int band_solve_tiled(hipStream_t s, const void *fac, int prec, int n, int bw, double *x, double *z);
 // path of a single-band factor / solve: 1 = multi-workgroup (tiled), 0 = one workgroup
 bool band_use_tiled_factor(int n, int bw, int prec);

@@ -102,5 +102,6 @@ def test_wide_band_takes_the_tiled_path(prec, ksh):
     assert neg == ref.negatives()[0]
     b = np.random.default_rng(8).standard_normal(o.nfree)
     xr = ref.solve(b)
-    x = K.solve(b) if prec == L.FP64 else K.solve(b, max_refine=30, tol=1e-13)
+    # FP32 factor of this nearly dense band: refinement contracts by ~0.6 per sweep
+    x = K.solve(b) if prec == L.FP64 else K.solve(b, max_refine=80, tol=1e-13)
     assert np.linalg.norm(x - xr) <= 1e-9 * np.linalg.norm(xr)
