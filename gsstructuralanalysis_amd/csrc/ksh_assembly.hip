@@ -9,8 +9,12 @@
 // Element tangent = batched B^T D B contraction (see ksh_shell_math.hpp for H_g):
 //     K_e^{ik}[A][B] = sum_{g,a} Psi_g[a][A] * ( sum_b H_g[(a,i),(b,k)] Psi_g[b][B] ),
 // one 16x16 output tile per component pair (i,k), inner dimension 5*ngp, on
-// v_mfma_f64_16x16x4f64 (FP64) or v_mfma_f32_16x16x4f32 (FP32).  Element tiles are gathered into
-// the band in a fixed element order (deterministic, no atomics).
+// v_mfma_f64_16x16x4f64 (FP64) or v_mfma_f32_16x16x4f32 (FP32).  H_g and the internal-force
+// vectors f_g come from one thread per Gauss point (state in registers); the element stages run
+// one wave per element.  Element matrices are stored dof-major (entry (3A+i, 3B+k) at
+// (3B+k)*48 + 3A+i), so the band gather's consecutive threads (consecutive rows of one band
+// column) read consecutive addresses; the gather sums in a fixed element order (deterministic,
+// no atomics).
 //
 // Every kernel runs over a batch of instances described by an InstDesc array (a single problem is
 // a batch of one): element kernels map blockIdx.x to (instance, element) and entry kernels map a
@@ -44,23 +48,6 @@ __device__ inline int find_inst(int n, long long key, Get get)
     return lo;
 }
 
-__device__ inline ElemArgs elem_args(const InstDesc &d)
-{
-    ElemArgs a;
-    a.p = d.p;
-    a.nx = d.nx;
-    a.ny = d.ny;
-    a.mx = d.mx;
-    a.my = d.ny + d.p;
-    a.t = d.t;
-    a.ctrl0 = d.ctrl0;
-    a.u_full = d.u_full;
-    a.tabx = d.tabx;
-    a.taby = d.taby;
-    a.gpgeo = d.gpgeo;
-    return a;
-}
-
 // ---------------------------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------------------------
@@ -74,49 +61,12 @@ __global__ void k_construct_solution(const InstDesc *descs, int ninst, int total
     d.u_full[g] = (f >= 0) ? d.u_free[f] : 0.0;
 }
 
+// deformed derivative vectors (a1, a2, a11, a22, a12) at one Gauss point of element (ex, ey)
 template <int P>
-__global__ __launch_bounds__(256) void k_residual_elem(const InstDesc *descs, int ninst)
+__device__ __forceinline__ void gp_derivatives(const InstDesc &d, int ex, int ey, const double *tx, const double *ty,
+                                               double (&dv)[15])
 {
-    constexpr int P1 = P + 1, NB = P1 * P1, NG = P1 * P1, NKK = ((5 * NG + 3) / 4) * 4;
-    __shared__ double sX[NB][3];
-    __shared__ double sPsi[NKK][16];
-    __shared__ double sDv[NG][15];
-    __shared__ double sSt[NG][shell::kState];
-    __shared__ double sF[NG][15];
-    const InstDesc &d = descs[find_inst(ninst, blockIdx.x, [&](int m) { return (long long)descs[m].elem0; })];
-    const int e = blockIdx.x - d.elem0, ex = e % d.nx, ey = e / d.nx;
-    ElemArgs ar = elem_args(d);
-    shell::element_states<P>(ar, ex, ey, sX, sPsi, sDv, sSt, NKK);
-    for (int idx = threadIdx.x; idx < NG * 15; idx += blockDim.x) {
-        int g = idx / 15, I = idx % 15;
-        sF[g][I] = shell::f_entry(sSt[g], I);
-    }
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < NB * 3; idx += blockDim.x) {
-        int A = idx / 3, i = idx % 3;
-        double acc = 0.0;
-        for (int g = 0; g < NG; ++g)
-            for (int a = 0; a < 5; ++a) acc += sPsi[5 * g + a][A] * sF[g][3 * a + i];
-        d.Fe[(size_t)e * 3 * NB + idx] = acc;
-    }
-}
-
-// Tangent, stage 1: one thread per Gauss point.  Derivative vectors of the deformed surface,
-// the point's state (shell::gp_state) and the 120 entries I <= J of its symmetric 15 x 15 H_g,
-// stored entry-major (Hg[q * ngp + gp]) so that a wave's stores are contiguous.  All indices
-// into the state are compile-time constants after unrolling, so it lives in registers.
-template <int P>
-__global__ __launch_bounds__(128) void k_tangent_gp(const InstDesc *descs, int ninst, int total)
-{
-    constexpr int P1 = P + 1, NB = P1 * P1, NQ = P1, NG = NQ * NQ;
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= total) return;
-    const InstDesc &d = descs[find_inst(ninst, gid, [&](int m) { return (long long)descs[m].elem0 * NG; })];
-    const int gp = gid - d.elem0 * NG;
-    const int e = gp / NG, g = gp % NG, ex = e % d.nx, ey = e / d.nx, qx = g % NQ, qy = g / NQ;
-    const double *tx = d.tabx + ((size_t)(ex * NQ + qx) * 3) * P1;
-    const double *ty = d.taby + ((size_t)(ey * NQ + qy) * 3) * P1;
-    double dv[15];
+    constexpr int P1 = P + 1, NB = P1 * P1;
 #pragma unroll
     for (int r = 0; r < 15; ++r) dv[r] = 0.0;
 #pragma unroll
@@ -131,6 +81,75 @@ __global__ __launch_bounds__(128) void k_tangent_gp(const InstDesc *descs, int n
             for (int a = 0; a < 5; ++a) dv[3 * a + c] += psi[a] * x;
         }
     }
+}
+
+// Residual, stage 1: one thread per Gauss point; its state and the 15 entries of f_g
+// (f_int_{A,i} = sum_g phi . f_g), stored entry-major in the H_g buffer (Fg[I * ngp + gp]).
+template <int P>
+__global__ __launch_bounds__(128) void k_residual_gp(const InstDesc *descs, int ninst, int total)
+{
+    constexpr int P1 = P + 1, NQ = P1, NG = NQ * NQ;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= total) return;
+    const InstDesc &d = descs[find_inst(ninst, gid, [&](int m) { return (long long)descs[m].elem0 * NG; })];
+    const int gp = gid - d.elem0 * NG;
+    const int e = gp / NG, g = gp % NG, ex = e % d.nx, ey = e / d.nx, qx = g % NQ, qy = g / NQ;
+    const double *tx = d.tabx + ((size_t)(ex * NQ + qx) * 3) * P1;
+    const double *ty = d.taby + ((size_t)(ey * NQ + qy) * 3) * P1;
+    double dv[15];
+    gp_derivatives<P>(d, ex, ey, tx, ty, dv);
+    double st[shell::kState];
+    shell::gp_state(dv, d.gpgeo + (size_t)gp * kGeoStride, d.t, st);
+    const size_t stride = (size_t)d.nel * NG;
+#pragma unroll
+    for (int I = 0; I < 15; ++I) d.Hg[(size_t)I * stride + gp] = shell::f_entry(st, I);
+}
+
+// Residual, stage 2: one wave per element; lane (A, i) sums Fe[3A+i] = sum_g sum_a
+// Psi_g[a][A] f_g[3a+i] (g outer, a inner).
+template <int P>
+__global__ __launch_bounds__(256) void k_residual_tiles(const InstDesc *descs, int ninst, int total_el)
+{
+    constexpr int P1 = P + 1, NB = P1 * P1, NQ = P1, NG = NQ * NQ;
+    const int eg = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    if (eg >= total_el) return;
+    const InstDesc &d = descs[find_inst(ninst, eg, [&](int m) { return (long long)descs[m].elem0; })];
+    const int e = eg - d.elem0, ex = e % d.nx, ey = e / d.nx;
+    const int lane = threadIdx.x & 63;
+    if (lane >= 3 * NB) return;
+    const int A = lane / 3, i = lane % 3, ia = A % P1, ib = A / P1;
+    const size_t stride = (size_t)d.nel * NG;
+    const double *Fg = d.Hg + (size_t)e * NG;
+    double acc = 0.0;
+    for (int g = 0; g < NG; ++g) {
+        const int qx = g % NQ, qy = g / NQ;
+        const double *tx = d.tabx + ((size_t)(ex * NQ + qx) * 3) * P1;
+        const double *ty = d.taby + ((size_t)(ey * NQ + qy) * 3) * P1;
+        const double psi[5] = {tx[P1 + ia] * ty[ib], tx[ia] * ty[P1 + ib], tx[2 * P1 + ia] * ty[ib],
+                               tx[ia] * ty[2 * P1 + ib], tx[P1 + ia] * ty[P1 + ib]};
+#pragma unroll
+        for (int a = 0; a < 5; ++a) acc += psi[a] * Fg[(size_t)(3 * a + i) * stride + g];
+    }
+    d.Fe[(size_t)e * 3 * NB + lane] = acc;
+}
+
+// Tangent, stage 1: one thread per Gauss point.  Derivative vectors of the deformed surface,
+// the point's state (shell::gp_state) and the 120 entries I <= J of its symmetric 15 x 15 H_g,
+// stored entry-major (Hg[q * ngp + gp]) so that a wave's stores are contiguous.  All indices
+// into the state are compile-time constants after unrolling, so it lives in registers.
+template <int P>
+__global__ __launch_bounds__(128) void k_tangent_gp(const InstDesc *descs, int ninst, int total)
+{
+    constexpr int P1 = P + 1, NQ = P1, NG = NQ * NQ;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= total) return;
+    const InstDesc &d = descs[find_inst(ninst, gid, [&](int m) { return (long long)descs[m].elem0 * NG; })];
+    const int gp = gid - d.elem0 * NG;
+    const int e = gp / NG, g = gp % NG, ex = e % d.nx, ey = e / d.nx, qx = g % NQ, qy = g / NQ;
+    const double *tx = d.tabx + ((size_t)(ex * NQ + qx) * 3) * P1;
+    const double *ty = d.taby + ((size_t)(ey * NQ + qy) * 3) * P1;
+    double dv[15];
+    gp_derivatives<P>(d, ex, ey, tx, ty, dv);
     double st[shell::kState];
     shell::gp_state(dv, d.gpgeo + (size_t)gp * kGeoStride, d.t, st);
     const size_t stride = (size_t)d.nel * NG;
@@ -197,6 +216,7 @@ __global__ __launch_bounds__(256) void k_tangent_tiles(const InstDesc *descs, in
                 acc[t] = M::run(aop, ok ? (T)z : T(0), acc[t]);  // padded steps add exact zeros
             }
     }
+    // element matrix, local dof 3A + i, column-major with leading dimension kKeLd
     T *Kout = (T *)d.Ke + (size_t)e * kKeTile;
     int t = 0;
 #pragma unroll
@@ -205,9 +225,9 @@ __global__ __launch_bounds__(256) void k_tangent_tiles(const InstDesc *descs, in
         for (int k = i; k < 3; ++k, ++t)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int row = M::row(lane, v);
-                Kout[(i * 3 + k) * 256 + row * 16 + li] = acc[t][v];
-                if (k != i) Kout[(k * 3 + i) * 256 + li * 16 + row] = acc[t][v];
+                const int row = M::row(lane, v);  // A; the lane's column li is B
+                Kout[(3 * li + k) * kKeLd + 3 * row + i] = acc[t][v];
+                if (k != i) Kout[(3 * row + i) * kKeLd + 3 * li + k] = acc[t][v];
             }
 }
 
@@ -237,7 +257,7 @@ __global__ __launch_bounds__(256) void k_mass_elem(const InstDesc *descs, int ni
         double acc = 0.0;
         for (int g = 0; g < NG; ++g) acc += d.rho * d.t * sN[g][A] * sN[g][B] * sW[g];
         for (int i = 0; i < 3; ++i)
-            for (int k = 0; k < 3; ++k) Kout[(i * 3 + k) * 256 + idx] = (i == k) ? (T)acc : (T)0;
+            for (int k = 0; k < 3; ++k) Kout[(3 * B + k) * kKeLd + 3 * A + i] = (i == k) ? (T)acc : (T)0;
     }
 }
 
@@ -356,7 +376,7 @@ __global__ void k_band_gather(const InstDesc *descs, int ninst, long long total)
             for (int ex = exlo; ex <= exhi; ++ex) {
                 int e = ex + d.nx * ey;
                 int A = (iI - ex) + P1 * (jI - ey), B = (iJ - ex) + P1 * (jJ - ey);
-                acc += (double)Ke[(size_t)e * kKeTile + (cI * 3 + cJ) * 256 + A * 16 + B];
+                acc += (double)Ke[(size_t)e * kKeTile + (3 * B + cJ) * kKeLd + 3 * A + cI];
             }
     }
     ((T *)d.ab)[slot] = (T)acc;
@@ -444,10 +464,16 @@ int run_residual(hipStream_t s, InstDesc *d_desc, std::vector<InstDesc> &h, bool
         hipLaunchKernelGGL(k_construct_solution, dim3(nblk(t.ndof, 256)), dim3(256), 0, s, d_desc, n, (int)t.ndof);
     {
         ProfScope ps("residual_elem", s);
-        if (h[0].p == 2)
-            hipLaunchKernelGGL(k_residual_elem<2>, dim3((unsigned)t.nel), dim3(256), 0, s, d_desc, n);
-        else
-            hipLaunchKernelGGL(k_residual_elem<3>, dim3((unsigned)t.nel), dim3(256), 0, s, d_desc, n);
+        const long long ng = (h[0].p + 1LL) * (h[0].p + 1LL), ngp = t.nel * ng;
+        if (ngp >= (1LL << 31)) return fail(KSH_OTHER_ERROR, "batch has 2^31 or more Gauss points");
+        const dim3 gg(nblk(ngp, 128)), ge(nblk(t.nel, 4));
+        if (h[0].p == 2) {
+            hipLaunchKernelGGL(k_residual_gp<2>, gg, dim3(128), 0, s, d_desc, n, (int)ngp);
+            hipLaunchKernelGGL(k_residual_tiles<2>, ge, dim3(256), 0, s, d_desc, n, (int)t.nel);
+        } else {
+            hipLaunchKernelGGL(k_residual_gp<3>, gg, dim3(128), 0, s, d_desc, n, (int)ngp);
+            hipLaunchKernelGGL(k_residual_tiles<3>, ge, dim3(256), 0, s, d_desc, n, (int)t.nel);
+        }
     }
     KSH_HIP(hipGetLastError());
     ProfScope ps("vector_gather", s);

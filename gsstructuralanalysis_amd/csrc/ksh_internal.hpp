@@ -11,7 +11,7 @@
 //                                 metric A_ab (3), curvature B_ab (3)
 //   u_full    [ndof]              scratch: constructSolution of the current free vector
 //   Hg        [120][nel*ngp]     per-Gauss-point H_g of the tangent (upper triangle, FP64)
-//   Ke        [nel][9][16][16]    element tangents in (component pair, A, B) tiles (FP64 or FP32)
+//   Ke        [nel][48][48]       element matrices, local dof 3A+i, column major (FP64 or FP32)
 //   Fe        [nel][3*nb]         element internal-force vectors
 //   F         [nfree]             external load (lambda = 1)
 // Band matrix: lower band, column major, ab[J*(bw+1) + (I-J)], FP64 or FP32.
@@ -30,7 +30,8 @@ namespace ksh {
 
 constexpr int kGeoStride = 16;   // doubles per Gauss point in gpgeo
 constexpr int kTile = 16;        // element tile edge (basis functions per element padded to 16)
-constexpr int kKeTile = 9 * kTile * kTile;
+constexpr int kKeTile = 9 * kTile * kTile;  // element matrix: 48 x 48 (3 components x 16 basis functions)
+constexpr int kKeLd = 3 * kTile;             // its leading dimension: entry (3A+i, 3B+k) at (3B+k) kKeLd + 3A+i
 constexpr int kHq = 120;         // stored entries of a Gauss point's symmetric 15x15 H_g
 
 void set_error(const std::string &msg);

@@ -208,55 +208,5 @@ struct ElemArgs {
     const double *ctrl0, *u_full, *tabx, *taby, *gpgeo;
 };
 
-// Psi[(g,a)][A] for a in {,1 ,2 ,11 ,22 ,12}; zero padded to [NKK][16]
-template <int P>
-__device__ void load_psi(const ElemArgs &ar, int ex, int ey, double (*sPsi)[16], int nkk)
-{
-    constexpr int P1 = P + 1, NB = P1 * P1, NQ = P1, NG = NQ * NQ;
-    for (int idx = threadIdx.x; idx < nkk * 16; idx += blockDim.x) {
-        int kk = idx / 16, A = idx % 16;
-        double v = 0.0;
-        if (kk < 5 * NG && A < NB) {
-            int g = kk / 5, a = kk % 5, qx = g % NQ, qy = g / NQ, ia = A % P1, ib = A / P1;
-            const double *tx = ar.tabx + ((size_t)(ex * NQ + qx) * 3) * P1;
-            const double *ty = ar.taby + ((size_t)(ey * NQ + qy) * 3) * P1;
-            switch (a) {
-            case 0: v = tx[P1 + ia] * ty[ib]; break;
-            case 1: v = tx[ia] * ty[P1 + ib]; break;
-            case 2: v = tx[2 * P1 + ia] * ty[ib]; break;
-            case 3: v = tx[ia] * ty[2 * P1 + ib]; break;
-            default: v = tx[P1 + ia] * ty[P1 + ib]; break;
-            }
-        }
-        sPsi[kk][A] = v;
-    }
-}
-
-// deformed control points + Psi + derivative vectors + per-GP state, shared by R and K kernels
-template <int P>
-__device__ void element_states(const ElemArgs &ar, int ex, int ey, double (*sX)[3], double (*sPsi)[16],
-                               double (*sDv)[15], double (*sSt)[kState], int nkk)
-{
-    constexpr int P1 = P + 1, NB = P1 * P1, NG = P1 * P1;
-    for (int idx = threadIdx.x; idx < NB * 3; idx += blockDim.x) {
-        int A = idx / 3, c = idx % 3, a = A % P1, b = A / P1;
-        int cp = (ex + a) + ar.mx * (ey + b);
-        sX[A][c] = ar.ctrl0[3 * cp + c] + ar.u_full[3 * cp + c];
-    }
-    load_psi<P>(ar, ex, ey, sPsi, nkk);
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < NG * 15; idx += blockDim.x) {
-        int g = idx / 15, r = idx % 15, a = r / 3, c = r % 3;
-        double acc = 0.0;
-        for (int A = 0; A < NB; ++A) acc += sPsi[5 * g + a][A] * sX[A][c];
-        sDv[g][r] = acc;
-    }
-    __syncthreads();
-    int e = ex + ar.nx * ey;
-    for (int g = threadIdx.x; g < NG; g += blockDim.x)
-        gp_state(sDv[g], ar.gpgeo + ((size_t)e * NG + g) * kGeoStride, ar.t, sSt[g]);
-    __syncthreads();
-}
-
 }  // namespace shell
 }  // namespace ksh
