@@ -74,6 +74,25 @@ __global__ void k_convert(TD *dst, const TS *src, size_t count)
     if (i < count) dst[i] = (TD)src[i];
 }
 
+// copy (or convert) of many bands in one launch: a 256-entry chunk per block, each block finds
+// its band by binary search over the chunk offsets
+__global__ void k_copy_batched(const CopyDesc *descs, int nband)
+{
+    const long long blk = blockIdx.x;
+    int lo = 0, hi = nband - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (descs[mid].blk0 <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    const CopyDesc &d = descs[lo];
+    const size_t i = (size_t)(blk - d.blk0) * 256 + threadIdx.x;
+    if (i >= d.count) return;
+    const double v = d.src_prec == KSH_FP64 ? ((const double *)d.src)[i] : (double)((const float *)d.src)[i];
+    if (d.dst_prec == KSH_FP64) ((double *)d.dst)[i] = v;
+    else ((float *)d.dst)[i] = (float)v;
+}
+
 __global__ void k_axpy(int n, double a, const double *x, double *y)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -158,6 +177,22 @@ int band_convert(const void *src, int src_prec, void *dst, int dst_prec, size_t 
                            count);
     else
         return fail(KSH_OTHER_ERROR, "band_convert: same precision");
+    KSH_HIP(hipGetLastError());
+    return KSH_SUCCESS;
+}
+
+int band_copy_batch(hipStream_t s, void *d_descs, const std::vector<CopyDesc> &hin)
+{
+    if (hin.empty()) return KSH_SUCCESS;
+    std::vector<CopyDesc> h = hin;
+    long long blocks = 0;
+    for (auto &d : h) {
+        d.blk0 = blocks;
+        blocks += (long long)((d.count + 255) / 256);
+    }
+    if (blocks >= (1LL << 31)) return fail(KSH_OTHER_ERROR, "band copy of 2^31 blocks or more");
+    KSH_HIP(hipMemcpyAsync(d_descs, h.data(), sizeof(CopyDesc) * h.size(), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_copy_batched, dim3((unsigned)blocks), dim3(256), 0, s, (const CopyDesc *)d_descs, (int)h.size());
     KSH_HIP(hipGetLastError());
     return KSH_SUCCESS;
 }

@@ -107,7 +107,7 @@ public:
         std::unique_lock<std::mutex> lk(m_mu);
         m_pending.push_back(&r);
         ++m_blocked;
-        m_cv_main.notify_one();
+        if (m_blocked == m_active) m_cv_main.notify_one();  // the engine waits for all of them
         m_cv_done.wait(lk, [&] { return r.done; });
         return r.rc;
     }
@@ -115,7 +115,7 @@ public:
     {
         std::lock_guard<std::mutex> lk(m_mu);
         --m_active;
-        m_cv_main.notify_one();
+        if (m_blocked == m_active) m_cv_main.notify_one();
     }
 
     // engine side ------------------------------------------------------------------------------
@@ -231,6 +231,7 @@ private:
         if ((rc = m_rdesc.ensure(n * sizeof(RefDesc)))) return rc;
         if ((rc = m_stats.ensure(n * 4 * sizeof(double)))) return rc;
         if ((rc = m_info.ensure(n * sizeof(int)))) return rc;
+        if ((rc = m_cdesc.ensure(n * sizeof(CopyDesc)))) return rc;
         if ((rc = m_norms.ensure(n * 2 * sizeof(double)))) return rc;
         return KSH_SUCCESS;
     }
@@ -255,6 +256,7 @@ private:
             }
             if (!h.empty() && (rc = batch_tangent(m_stream, (InstDesc *)m_desc.p, h))) return rc;
         }
+        std::vector<Matrix *> ms;
         for (size_t i = 0; i < g.size(); ++i) {
             Matrix &K = *g[i]->K;
             if (K.sprec == KSH_FP32) {
@@ -264,8 +266,13 @@ private:
             }
             ++K.version;
             g[i]->rc = KSH_SUCCESS;
+            ms.push_back(&K);
         }
-        return KSH_SUCCESS;
+        // The solvers factor every tangent they assemble right after assembling it
+        // (gsALMBase::computeJacobian, src/gsALMSolvers/gsALMBase.hpp:186-211), so the factor
+        // goes into this round; the instance's factor request then finds the matrix factored
+        // and returns without a round of its own.
+        return factor_matrices(ms);
     }
 
     int do_residual(std::vector<Req *> &g)
@@ -322,58 +329,60 @@ private:
 
     int do_factor(std::vector<Req *> &g)
     {
-        int rc;
-        if ((rc = ensure_desc(g.size()))) return rc;
-        std::vector<Req *> todo;
+        std::vector<Matrix *> todo;
         for (Req *r : g) {
             Matrix &K = *r->K;
-            if (K.factored && K.factored_version == K.version) r->rc = K.info ? KSH_SOLVER_ERROR : KSH_SUCCESS;
-            else todo.push_back(r);
+            if (!(K.factored && K.factored_version == K.version)) todo.push_back(&K);
         }
-        if (todo.empty()) return KSH_SUCCESS;
+        int rc;
+        if ((rc = factor_matrices(todo))) return rc;
+        for (Req *r : g)
+            r->rc = r->K->info ? fail(KSH_SOLVER_ERROR, "LDL^T: zero or non-finite pivot") : KSH_SUCCESS;
+        return KSH_SUCCESS;
+    }
+
+    // LDL^T of the current values of several matrices: one batched copy into the factor storage,
+    // one batched factor launch per factor precision, and the FP64 refactorization of the FP32
+    // factors whose smallest pivot is too small to trust (one more batched launch).  The
+    // outcome is in each matrix (factored_version, info, pivot statistics, use64).
+    int factor_matrices(const std::vector<Matrix *> &ms)
+    {
+        if (ms.empty()) return KSH_SUCCESS;
+        int rc;
+        if ((rc = ensure_desc(ms.size()))) return rc;
         for (int fp : {KSH_FP64, KSH_FP32}) {
             std::vector<BandDescHost> h;
-            std::vector<Req *> grp;
-            for (Req *r : todo) {
-                Matrix &K = *r->K;
-                if (K.fprec != fp) continue;
-                size_t count = (size_t)K.n * K.ld;
-                if (K.sprec == fp) {
-                    KSH_HIP(hipMemcpyAsync(K.fac, K.ab, count * (fp == KSH_FP64 ? 8 : 4), hipMemcpyDeviceToDevice, m_stream));
-                } else if ((rc = band_convert(K.ab, K.sprec, K.fac, fp, count, m_stream))) {
-                    return rc;
-                }
-                size_t k = grp.size();
-                h.push_back(BandDescHost{K.fac, K.n, K.bw, (int *)m_info.p + k, (double *)m_stats.p + 4 * k, nullptr, 1});
-                grp.push_back(r);
+            std::vector<CopyDesc> cp;
+            std::vector<Matrix *> grp;
+            for (Matrix *K : ms) {
+                if (K->fprec != fp) continue;
+                cp.push_back(CopyDesc{K->ab, K->fac, (size_t)K->n * K->ld, K->sprec, fp, 0});
+                const size_t k = grp.size();
+                h.push_back(BandDescHost{K->fac, K->n, K->bw, (int *)m_info.p + k, (double *)m_stats.p + 4 * k, nullptr, 1});
+                grp.push_back(K);
             }
             if (grp.empty()) continue;
+            if ((rc = band_copy_batch(m_stream, m_cdesc.p, cp))) return rc;
             if ((rc = band_factor_batch(m_stream, m_bdesc.p, h, fp))) return rc;
             std::vector<int> info(grp.size());
             std::vector<double> st(4 * grp.size());
             KSH_HIP(hipMemcpyAsync(info.data(), m_info.p, sizeof(int) * grp.size(), hipMemcpyDeviceToHost, m_stream));
             KSH_HIP(hipMemcpyAsync(st.data(), m_stats.p, sizeof(double) * 4 * grp.size(), hipMemcpyDeviceToHost, m_stream));
             KSH_HIP(hipStreamSynchronize(m_stream));
+            std::vector<Matrix *> re;
             for (size_t k = 0; k < grp.size(); ++k) {
-                Matrix &K = *grp[k]->K;
+                Matrix &K = *grp[k];
                 K.factored = true;
                 K.factored_version = K.version;
                 K.use64 = false;
                 K.info = info[k];
                 for (int q = 0; q < 4; ++q) K.fstats[q] = st[4 * k + q];
                 K.has_stats = info[k] == 0;
-            }
-            // FP32 factors with a pivot too small to trust (or a failed one): FP64 refactorization
-            // of the same matrices, all in one batched launch
-            std::vector<Matrix *> re;
-            for (size_t k = 0; k < grp.size(); ++k) {
-                Matrix &K = *grp[k]->K;
+                // FP32 factors with a pivot too small to trust (or a failed one) are redone in FP64
                 if (fp == KSH_FP32 && (info[k] != 0 || !(K.fstats[3] >= K.guard * K.fstats[2]))) re.push_back(&K);
             }
             prof_count("fp64_refactor_guard", (long long)re.size());
             if ((rc = refactor64_batch(re))) return rc;
-            for (size_t k = 0; k < grp.size(); ++k)
-                grp[k]->rc = grp[k]->K->info ? fail(KSH_SOLVER_ERROR, "LDL^T: zero or non-finite pivot") : KSH_SUCCESS;
         }
         return KSH_SUCCESS;
     }
@@ -388,21 +397,20 @@ private:
         int rc;
         if ((rc = ensure_desc(ms.size()))) return rc;
         std::vector<InstDesc> tang;
+        std::vector<CopyDesc> cp;
         for (Matrix *K : ms) {
-            const size_t count = (size_t)K->n * K->ld;
             if ((rc = ensure_fac64(*K))) return rc;
-            if (K->sprec == KSH_FP64) {
-                KSH_HIP(hipMemcpyAsync(K->fac64, K->ab, count * sizeof(double), hipMemcpyDeviceToDevice, m_stream));
-            } else if (K->has_u) {
+            if (K->sprec == KSH_FP32 && K->has_u) {
                 InstDesc d = make_desc(*K->pb);
                 d.u_free = K->u_at;
                 d.ab = K->fac64;
                 d.sprec = KSH_FP64;
                 tang.push_back(d);
-            } else if ((rc = band_convert(K->ab, K->sprec, K->fac64, KSH_FP64, count, m_stream))) {
-                return rc;
+            } else {
+                cp.push_back(CopyDesc{K->ab, K->fac64, (size_t)K->n * K->ld, K->sprec, KSH_FP64, 0});
             }
         }
+        if ((rc = band_copy_batch(m_stream, m_cdesc.p, cp))) return rc;
         if (!tang.empty() && (rc = batch_tangent(m_stream, (InstDesc *)m_desc.p, tang))) return rc;
         std::vector<BandDescHost> h;
         for (size_t k = 0; k < ms.size(); ++k)
@@ -556,7 +564,7 @@ private:
     long long m_refactors = 0, m_refactor_launches = 0;  // batched FP64 refactorizations (matrices, launches)
     std::vector<Req *> m_pending;
     hipStream_t m_stream = nullptr;
-    DevBuf m_din[3], m_dout, m_dr, m_desc, m_bdesc, m_rdesc, m_stats, m_info, m_norms, m_part;
+    DevBuf m_din[3], m_dout, m_dr, m_desc, m_bdesc, m_cdesc, m_rdesc, m_stats, m_info, m_norms, m_part;
     HostBuf m_hin[3], m_hout;
     size_t m_tot = 0;
 };
@@ -585,6 +593,8 @@ public:
     }
     int factor(ksh_matrix *K) override
     {
+        // factored by the engine in the round that assembled it (or by an earlier request)
+        if (K->factored && K->factored_version == K->version) return K->info ? KSH_SOLVER_ERROR : KSH_SUCCESS;
         Req r{RQ_FACTOR};
         r.pb = K->pb == nullptr ? nullptr : static_cast<ksh_problem *>(K->pb);
         r.K = K;

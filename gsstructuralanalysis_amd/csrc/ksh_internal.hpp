@@ -192,6 +192,16 @@ int band_inertia(Matrix &K, const void *fac, int prec, int64_t *neg, double *dmi
                  double *dminabs);
 int band_axpby(Matrix &Y, double alpha, double beta, const Matrix &X);
 int band_convert(const void *src, int src_prec, void *dst, int dst_prec, size_t count, hipStream_t s);
+// copies (with conversion between FP64 and FP32 where the precisions differ) of many bands in
+// one launch; d_descs: device room for h.size() CopyDescs
+struct CopyDesc {
+    const void *src;
+    void *dst;
+    size_t count;      // entries
+    int src_prec, dst_prec;
+    long long blk0;    // set by the launcher
+};
+int band_copy_batch(hipStream_t s, void *d_descs, const std::vector<CopyDesc> &h);
 int vec_axpy(hipStream_t s, int n, double a, const double *x, double *y);
 int vec_sub(hipStream_t s, int n, const double *a, const double *b, double *out);
 int vec_norm2(hipStream_t s, int n, const double *x, double *scratch, double *host_out);
