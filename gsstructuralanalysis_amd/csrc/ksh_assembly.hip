@@ -168,11 +168,12 @@ __device__ __forceinline__ int hq_index(int I, int J)
     return r * 15 - (r * (r - 1)) / 2 + (c - r);
 }
 
-// Tangent, stage 2: one wave per element.  K_e^{ik} = Psi^T Z^{ik} on 16 x 16 MFMA tiles
+// Tangent, stage 2: one wave per element.  The element's H_g (NG x 120 entries) is read once,
+// coalesced, into the wave's part of LDS.  K_e^{ik} = Psi^T Z^{ik} on 16 x 16 MFMA tiles
 // (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32), inner dimension kk = (g, a) over 5 NG; the
 // B operand Z^{ik}[kk][B] = sum_b H_g[(a,i),(b,k)] Psi_g[b][B] is formed in the lane that feeds
-// it, from H_g (L2) and basis values (tables, L1).  The six tiles i <= k are computed; the
-// tiles i > k are stored as their transposes.
+// it, from H_g (LDS broadcasts) and basis values (tables, L1).  The six tiles i <= k are
+// computed; the tiles i > k are stored as their transposes.
 template <int P, typename T>
 __global__ __launch_bounds__(256) void k_tangent_tiles(const InstDesc *descs, int ninst, int total_el)
 {
@@ -186,7 +187,19 @@ __global__ __launch_bounds__(256) void k_tangent_tiles(const InstDesc *descs, in
     const int ia = li % P1, ib = li / P1;
     const bool inA = li < NB;
     const size_t stride = (size_t)d.nel * NG;
-    const double *Hb = d.Hg + (size_t)e * NG;
+    constexpr int kHs = kHq + 1;  // padded row of the LDS copy
+    __shared__ double sH[4][NG][kHs];
+    double(*sHw)[kHs] = sH[threadIdx.x >> 6];
+    {
+        const double *Hb = d.Hg + (size_t)e * NG;
+        for (int idx = lane; idx < NG * kHq; idx += 64) {
+            const int q = idx / NG, g = idx % NG;  // NG consecutive points of one entry: contiguous
+            sHw[g][q] = Hb[(size_t)q * stride + g];
+        }
+        // the wave reads what its own lanes wrote: the LDS writes must be complete
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    }
     typename M::acc_t acc[6];
 #pragma unroll
     for (int t = 0; t < 6; ++t) acc[t] = typename M::acc_t{0, 0, 0, 0};
@@ -204,7 +217,7 @@ __global__ __launch_bounds__(256) void k_tangent_tiles(const InstDesc *descs, in
         psi[4] = ok ? tx[P1 + ia] * ty[P1 + ib] : 0.0;
         const double pa = a == 0 ? psi[0] : a == 1 ? psi[1] : a == 2 ? psi[2] : a == 3 ? psi[3] : psi[4];
         const T aop = (T)pa;
-        const double *Hg = Hb + g;
+        const double *Hg = sHw[g];
         int t = 0;
 #pragma unroll
         for (int i = 0; i < 3; ++i)
@@ -212,7 +225,7 @@ __global__ __launch_bounds__(256) void k_tangent_tiles(const InstDesc *descs, in
             for (int k = i; k < 3; ++k, ++t) {
                 double z = 0.0;
 #pragma unroll
-                for (int b = 0; b < 5; ++b) z += Hg[(size_t)hq_index(3 * a + i, 3 * b + k) * stride] * psi[b];
+                for (int b = 0; b < 5; ++b) z += Hg[hq_index(3 * a + i, 3 * b + k)] * psi[b];
                 acc[t] = M::run(aop, ok ? (T)z : T(0), acc[t]);  // padded steps add exact zeros
             }
     }
