@@ -23,6 +23,7 @@
 
 #include <cmath>
 
+#include "ksh_devutil.hpp"
 #include "ksh_internal.hpp"
 #include "ksh_mfma.hpp"
 #include "ksh_profile.hpp"
@@ -366,32 +367,43 @@ __global__ void k_vector_gather(const InstDesc *descs, int ninst, int total)
     d.R[I] = acc;
 }
 
-// Deterministic gather of element tiles into the lower band: one thread per band slot.
-template <typename T>
+// Deterministic gather of element matrices into the lower band: one thread per band slot sums
+// the (p+1)^2 candidate elements that could hold both control points, in a fixed order (ey, then
+// ex, ascending); candidates outside the patch or not sharing both points are masked loads (0 from
+// beyond the buffer resource), so every thread issues the same straight-line loads.
+template <int P, typename T>
 __global__ void k_band_gather(const InstDesc *descs, int ninst, long long total)
 {
+    constexpr int P1 = P + 1;
     long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= total) return;
     const InstDesc &d = descs[find_inst(ninst, gid, [&](int m) { return descs[m].slot0; })];
     long long slot = gid - d.slot0;
     const long long ld = (long long)d.bw + 1;
     int J = (int)(slot / ld), dd = (int)(slot % ld), I = J + dd;
+    const bool row_ok = I < d.nfree;
+    const int gI = d.free2full[row_ok ? I : J], gJ = d.free2full[J];
+    const int cpI = gI / 3, cI = gI % 3, cpJ = gJ / 3, cJ = gJ % 3;
+    const int iI = cpI % d.mx, jI = cpI / d.mx, iJ = cpJ % d.mx, jJ = cpJ / d.mx;
+    const int ex0 = max(iI, iJ) - P, exhi = min(min(iI, iJ), d.nx - 1);
+    const int ey0 = max(jI, jJ) - P, eyhi = min(min(jI, jJ), d.ny - 1);
+    const __amdgpu_buffer_rsrc_t rs = band_rsrc(d.Ke, (unsigned)((size_t)d.nel * kKeTile * sizeof(T)));
+    T v[P1][P1];
+#pragma unroll
+    for (int ty = 0; ty < P1; ++ty)
+#pragma unroll
+        for (int tx = 0; tx < P1; ++tx) {
+            const int ex = ex0 + tx, ey = ey0 + ty;
+            const bool ok = row_ok && ex >= 0 && ex <= exhi && ey >= 0 && ey <= eyhi;
+            const int A = (iI - ex) + P1 * (jI - ey), B = (iJ - ex) + P1 * (jJ - ey);
+            const unsigned off = (unsigned)(((size_t)(ex + d.nx * ey) * kKeTile + (3 * B + cJ) * kKeLd + 3 * A + cI) * sizeof(T));
+            v[ty][tx] = load_if<T>(ok, rs, off);
+        }
     double acc = 0.0;
-    if (I < d.nfree) {
-        int p = d.p, P1 = p + 1;
-        int gI = d.free2full[I], gJ = d.free2full[J];
-        int cpI = gI / 3, cI = gI % 3, cpJ = gJ / 3, cJ = gJ % 3;
-        int iI = cpI % d.mx, jI = cpI / d.mx, iJ = cpJ % d.mx, jJ = cpJ / d.mx;
-        int exlo = max(max(iI, iJ) - p, 0), exhi = min(min(iI, iJ), d.nx - 1);
-        int eylo = max(max(jI, jJ) - p, 0), eyhi = min(min(jI, jJ), d.ny - 1);
-        const T *Ke = (const T *)d.Ke;
-        for (int ey = eylo; ey <= eyhi; ++ey)
-            for (int ex = exlo; ex <= exhi; ++ex) {
-                int e = ex + d.nx * ey;
-                int A = (iI - ex) + P1 * (jI - ey), B = (iJ - ex) + P1 * (jJ - ey);
-                acc += (double)Ke[(size_t)e * kKeTile + (3 * B + cJ) * kKeLd + 3 * A + cI];
-            }
-    }
+#pragma unroll
+    for (int ty = 0; ty < P1; ++ty)
+#pragma unroll
+        for (int tx = 0; tx < P1; ++tx) acc += (double)v[ty][tx];
     ((T *)d.ab)[slot] = (T)acc;
 }
 
@@ -462,6 +474,8 @@ int check_batch(const std::vector<InstDesc> &h, bool band)
     for (auto &d : h) {
         if (d.p != h[0].p) return fail(KSH_OTHER_ERROR, "batch mixes spline degrees");
         if (band && d.sprec != h[0].sprec) return fail(KSH_OTHER_ERROR, "batch mixes band precisions");
+        if (band && (size_t)d.nel * kKeTile * sizeof(double) >= 0x80000000ull)
+            return fail(KSH_OTHER_ERROR, "element matrices of 2 GiB or more: beyond the gather's 32-bit offsets");
     }
     return KSH_SUCCESS;
 }
@@ -541,8 +555,11 @@ int run_tangent(hipStream_t s, InstDesc *d_desc, std::vector<InstDesc> &h, bool 
         for (const auto &d : h) work += (double)d.nel * kKeTile * es + (double)d.nfree * (d.bw + 1.0) * es;
         ps.set_work(work);
     }
-    if (f64) hipLaunchKernelGGL(k_band_gather<double>, dim3(nblk(t.slots, 256)), b, 0, s, d_desc, n, t.slots);
-    else hipLaunchKernelGGL(k_band_gather<float>, dim3(nblk(t.slots, 256)), b, 0, s, d_desc, n, t.slots);
+    const dim3 gs(nblk(t.slots, 256));
+    if (p2 && f64) hipLaunchKernelGGL((k_band_gather<2, double>), gs, b, 0, s, d_desc, n, t.slots);
+    else if (p2) hipLaunchKernelGGL((k_band_gather<2, float>), gs, b, 0, s, d_desc, n, t.slots);
+    else if (f64) hipLaunchKernelGGL((k_band_gather<3, double>), gs, b, 0, s, d_desc, n, t.slots);
+    else hipLaunchKernelGGL((k_band_gather<3, float>), gs, b, 0, s, d_desc, n, t.slots);
     KSH_HIP(hipGetLastError());
     return KSH_SUCCESS;
 }
