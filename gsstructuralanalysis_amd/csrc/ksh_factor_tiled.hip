@@ -10,12 +10,13 @@
 // reproduces the count gsALMBase::_computeStability takes (src/gsALMSolvers/gsALMHelper.h:30-42).
 //
 // Layout: lower band, column major, fac[j*ld + (i-j)], ld = bw+1, factored in place into unit L
-// and D.  Right-looking in panels of NB = 32 columns.  Panel step at c0:
-//   1. k_band_panel (one workgroup of 512 threads per band): the NB x NB diagonal block is
-//      staged in LDS and wave 0 factors it (lane = row; column jj is shared through LDS) and
-//      writes it back; then each thread solves rows below it against L_dd D (row-wise TRSM),
-//      read and written straight from the band.  (One workgroup per band: a second one would
-//      read the diagonal block while the first writes it back.)
+// and D.  Right-looking in panels of NB = 64 columns.  Panel step at c0:
+//   1. k_band_panel (one workgroup of 512 threads per band), two sub-panels of 32 columns: the
+//      32 x 32 diagonal block is staged in LDS, wave 0 factors it (lane = row; column jj is
+//      shared through LDS) and writes it back, each thread solves rows below it against L_dd D
+//      (row-wise TRSM) straight from the band; between the two sub-panels the second one's
+//      columns take the first one's rank-32 update.  (One workgroup per band: a second one
+//      would read the diagonal block while the first writes it back.)
 //   2. k_band_trailing (per band a workgroup of 4 waves per 4 blocks): the lower triangle of the
 //      bw x bw trailing window is cut into 32 x 32 blocks, one per wave, each a 2 x 2 set of
 //      16 x 16 MFMA tiles (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32) over the NB panel
@@ -39,93 +40,142 @@ namespace ksh {
 
 namespace {
 
-constexpr int kNB = 32;   // panel width
-constexpr int kPT = 512;  // panel kernel threads (TRSM rows per pass)
-constexpr int kUW = 4;    // waves per trailing-update workgroup
+constexpr int kNS = 32;       // sub-panel width (diagonal block factored by one wave)
+constexpr int kNB = 2 * kNS;  // panel width (rank of one trailing update)
+constexpr int kPT = 512;      // panel kernel threads (TRSM rows per pass)
+constexpr int kUW = 4;        // waves per trailing-update workgroup
 
+// Sub-panel at column cs (width ws = min(32, n - cs)): factor the diagonal block in LDS (wave 0),
+// write it back, and solve the rows below it within the band (row-wise TRSM).  Rows cs+32 ..
+// cs+63 of the result are also left in `lnext` (for the next sub-panel's update).  The panel's
+// arithmetic is FP64 for both storage precisions (an FP32 band is read, and its factor written,
+// in FP32; the trailing update, almost all of the work, runs in the storage precision).
 template <typename T>
-__global__ __launch_bounds__(kPT) void k_band_panel(const BandDesc *descs, int c0)
+__device__ __forceinline__ void sub_panel(const BandDesc &d, __amdgpu_buffer_rsrc_t rs, int cs, double *blk, double *dinv,
+                                          double *dval, double (*colv)[kNS], double (*lnext)[kNS + 1])
 {
-    __shared__ T blk[kNB * kNB];  // blk[kk*NB + r] = A / L[c0+r][c0+kk] for r >= kk, zero above
-    __shared__ T dinv[kNB];
-    __shared__ T colv[2][kNB];    // column of the current elimination step (double-buffered)
-    const BandDesc &d = descs[blockIdx.x];
     const int n = d.n, bw = d.bw;
-    if (c0 >= n) return;
-    const int w = min(kNB, n - c0), nr = min(n - c0, w + bw);
-    T *fac = (T *)d.fac;
+    const int w = min(kNS, n - cs), nr = min(n - cs, w + bw);
     const unsigned ldu = (unsigned)bw + 1;
     constexpr unsigned es = sizeof(T);
-    const __amdgpu_buffer_rsrc_t rs = band_rsrc(fac, (unsigned)((size_t)n * ldu * es));
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    constexpr int kQ = kNB * kNB / kPT;
+    constexpr int kQ = kNS * kNS / kPT;
     // stage the diagonal block (unit pivots pad the last panel)
 #pragma unroll
     for (int q = 0; q < kQ; ++q) {
-        const int idx = tid + kPT * q, kk = idx / kNB, r = idx % kNB;
+        const int idx = tid + kPT * q, kk = idx / kNS, r = idx % kNS;
         const bool ok = r >= kk && r < w && r - kk <= bw;
-        const T v = load_if<T>(ok, rs, ((unsigned)(c0 + kk) * ldu + (unsigned)(r - kk)) * es);
-        blk[idx] = (r >= w && r == kk) ? T(1) : v;
+        const double v = (double)load_if<T>(ok, rs, ((unsigned)(cs + kk) * ldu + (unsigned)(r - kk)) * es);
+        blk[idx] = (r >= w && r == kk) ? 1.0 : v;
     }
     __syncthreads();
     if (wave == 0) {
-        // lane r owns row c0+r.  Step jj shares column jj through LDS (every lane reads the same
-        // entries: broadcasts).  Entries above the diagonal take updates too; never read.
-        const int r = lane & (kNB - 1);
-        T a[kNB];
+        // lane r owns row cs+r (lanes >= 32 shadow row r-32).  Step jj shares column jj through
+        // LDS (every lane reads the same entries: broadcasts).  Entries above the diagonal take
+        // updates too; never read.
+        const int r = lane & (kNS - 1);
+        double a[kNS];
 #pragma unroll
-        for (int kk = 0; kk < kNB; ++kk) a[kk] = blk[kk * kNB + r];
+        for (int kk = 0; kk < kNS; ++kk) a[kk] = blk[kk * kNS + r];
         int bad = 0;
 #pragma unroll
-        for (int jj = 0; jj < kNB; ++jj) {
-            T *col = colv[jj & 1];
+        for (int jj = 0; jj < kNS; ++jj) {
+            double *col = colv[jj & 1];
             col[r] = a[jj];
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-            const T djj = col[jj];
-            if (jj < w && bad == 0 && (djj == T(0) || !isfinite((double)djj))) bad = c0 + jj + 1;
-            const T l = a[jj] / djj;
+            const double djj = col[jj];
+            if (jj < w && bad == 0 && (djj == 0.0 || !isfinite(djj) || !isfinite((double)(T)djj))) bad = cs + jj + 1;
+            const double l = a[jj] / djj;
 #pragma unroll
-            for (int kk = jj + 1; kk < kNB; ++kk) a[kk] -= l * col[kk];
+            for (int kk = jj + 1; kk < kNS; ++kk) a[kk] -= l * col[kk];
             if (r > jj) a[jj] = l;
             __builtin_amdgcn_sched_barrier(0);  // keeps the steps' LDS reads from being hoisted together
         }
-        // first bad pivot of the factorization: the first panel clears the flag
-        if (lane == 0 && d.info && (c0 == 0 || (bad != 0 && *d.info == 0))) *d.info = bad;
-        dinv[r] = T(1) / a[r];
+        // first bad pivot of the factorization: the first sub-panel clears the flag
+        if (lane == 0 && d.info && (cs == 0 || (bad != 0 && *d.info == 0))) *d.info = bad;
+        dval[r] = a[r];
+        dinv[r] = 1.0 / a[r];
 #pragma unroll
-        for (int kk = 0; kk < kNB; ++kk) blk[kk * kNB + r] = kk <= r ? a[kk] : T(0);
+        for (int kk = 0; kk < kNS; ++kk) blk[kk * kNS + r] = kk <= r ? a[kk] : 0.0;
     }
     __syncthreads();
     // write the factored block back (L below the diagonal, D on it)
 #pragma unroll
     for (int q = 0; q < kQ; ++q) {
-        const int idx = tid + kPT * q, kk = idx / kNB, r = idx % kNB;
+        const int idx = tid + kPT * q, kk = idx / kNS, r = idx % kNS;
         const bool ok = r >= kk && r < w && r - kk <= bw;
-        store_if<T>(ok, rs, ((unsigned)(c0 + kk) * ldu + (unsigned)(r - kk)) * es, blk[idx]);
+        store_if<T>(ok, rs, ((unsigned)(cs + kk) * ldu + (unsigned)(r - kk)) * es, (T)blk[idx]);
     }
-    // TRSM: row r of the panel, L[r][0:NB] = A[r][0:NB] (D L_dd^T)^-1
+    // TRSM: row r of the sub-panel, L[r][0:32] = A[r][0:32] (D L_dd^T)^-1
     for (int r = w + tid; r < nr; r += kPT) {
-        T a[kNB];
+        double a[kNS];
 #pragma unroll
-        for (int kk = 0; kk < kNB; ++kk) {
+        for (int kk = 0; kk < kNS; ++kk) {
             const bool ok = kk < w && r - kk <= bw;
-            a[kk] = load_if<T>(ok, rs, ((unsigned)(c0 + kk) * ldu + (unsigned)(r - kk)) * es);
+            a[kk] = (double)load_if<T>(ok, rs, ((unsigned)(cs + kk) * ldu + (unsigned)(r - kk)) * es);
         }
 #pragma unroll
-        for (int kk = 0; kk < kNB; ++kk) {
-            const T t = a[kk];  // = L[r][kk] d_kk
+        for (int kk = 0; kk < kNS; ++kk) {
+            const double t = a[kk];  // = L[r][kk] d_kk
 #pragma unroll
-            for (int jj = kk + 1; jj < kNB; ++jj) a[jj] -= t * blk[kk * kNB + jj];
+            for (int jj = kk + 1; jj < kNS; ++jj) a[jj] -= t * blk[kk * kNS + jj];
             a[kk] = t * dinv[kk];
             __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
-        for (int kk = 0; kk < kNB; ++kk) {
+        for (int kk = 0; kk < kNS; ++kk) {
             const bool ok = kk < w && r - kk <= bw;
-            store_if<T>(ok, rs, ((unsigned)(c0 + kk) * ldu + (unsigned)(r - kk)) * es, a[kk]);
+            store_if<T>(ok, rs, ((unsigned)(cs + kk) * ldu + (unsigned)(r - kk)) * es, (T)a[kk]);
+        }
+        if (r >= kNS && r < 2 * kNS) {
+#pragma unroll
+            for (int kk = 0; kk < kNS; ++kk) lnext[r - kNS][kk] = a[kk];
         }
     }
+}
+
+// Panel of 64 columns at c0 as two sub-panels of 32: the second sub-panel's columns take the
+// first one's rank-32 update (A[r][j] -= sum_k L[r][k] d_k L[j][k]) between the two.
+template <typename T>
+__global__ __launch_bounds__(kPT) void k_band_panel(const BandDesc *descs, int c0)
+{
+    __shared__ double blk[kNS * kNS];     // blk[kk*32 + r] = A / L[cs+r][cs+kk] for r >= kk, zero above
+    __shared__ double dinv[kNS], dval[kNS];
+    __shared__ double colv[2][kNS];       // column of the current elimination step (double-buffered)
+    __shared__ double lnext[kNS][kNS + 1];  // L[c0+32+j][c0+k] of the first sub-panel
+    const BandDesc &d = descs[blockIdx.x];
+    const int n = d.n, bw = d.bw;
+    if (c0 >= n) return;
+    T *fac = (T *)d.fac;
+    const unsigned ldu = (unsigned)bw + 1;
+    constexpr unsigned es = sizeof(T);
+    const __amdgpu_buffer_rsrc_t rs = band_rsrc(fac, (unsigned)((size_t)n * ldu * es));
+    sub_panel<T>(d, rs, c0, blk, dinv, dval, colv, lnext);
+    const int c1 = c0 + kNS;
+    if (c1 >= n) return;  // uniform: the last panel has one sub-panel
+    const int w0 = kNS, w1 = min(kNS, n - c1), nr0 = min(n - c0, w0 + bw);
+    __syncthreads();  // lnext complete, TRSM stores visible
+    // update of columns c1 .. c1+w1-1, rows r (relative to c0) in [32, nr0)
+    for (int r = kNS + (int)threadIdx.x; r < nr0; r += kPT) {
+        double l[kNS];
+#pragma unroll
+        for (int kk = 0; kk < kNS; ++kk) {
+            const bool ok = r - kk <= bw;
+            l[kk] = (double)load_if<T>(ok, rs, ((unsigned)(c0 + kk) * ldu + (unsigned)(r - kk)) * es) * dval[kk];
+        }
+        for (int j = 0; j < w1; ++j) {
+            const int col = kNS + j;  // relative to c0
+            const bool ok = r >= col && r - col <= bw;
+            const unsigned off = ((unsigned)(c0 + col) * ldu + (unsigned)(r - col)) * es;
+            double acc = (double)load_if<T>(ok, rs, off);
+#pragma unroll
+            for (int kk = 0; kk < kNS; ++kk) acc -= l[kk] * lnext[j][kk];
+            store_if<T>(ok, rs, off, (T)acc);
+        }
+    }
+    __syncthreads();  // updated columns visible to the second sub-panel's loads
+    sub_panel<T>(d, rs, c1, blk, dinv, dval, colv, lnext);
 }
 
 // lower-triangle block index p -> (bi, bj), bj <= bi
