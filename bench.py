@@ -135,7 +135,8 @@ def main():
     ap.add_argument("--form", choices=["mixed+learned", "mixed", "batched+learned", "batched"], default="mixed+learned",
                     help="headline form")
     ap.add_argument("--epochs", type=int, default=150, help="training epochs of the learned predictor")
-    ap.add_argument("--only", action="store_true", help="time the headline form only")
+    ap.add_argument("--only", action="store_true",
+                    help="time the headline form only (no other forms, no held-out predictor evaluation)")
     ap.add_argument("--also", nargs="*", default=None, metavar="FORM",
                     help="the other forms to time (default: all); compared with the FP64 batched form")
     ap.add_argument("--max-refine", type=int, default=3, help="mixed form: refinement sweeps per solve")
@@ -290,7 +291,7 @@ def main():
 
     # held-out evaluation of the predictor: tangent vs learned on 64 roofs never trained on
     heldout = None
-    if net is not None and len(sps) > 1:
+    if net is not None and len(sps) > 1 and not args.only:
         prec = forms[args.form] if args.form.endswith("+learned") else forms["mixed+learned"]
         base_kw = {k: v for k, v in prec.items() if k != "learned"}
         ho = [ShellProblem(g.imperfect_roof_instance(i, seed0=30261202)) for i in range(64)]
