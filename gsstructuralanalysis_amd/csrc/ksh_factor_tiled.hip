@@ -178,6 +178,164 @@ __global__ __launch_bounds__(kPT) void k_band_panel(const BandDesc *descs, int c
     sub_panel<T>(d, rs, c1, blk, dinv, dval, colv, lnext);
 }
 
+// Factor of a 32 x 32 diagonal block held in LDS (blk[kk*32 + r], lower triangle, unit pivots
+// past w) by wave 0: lane r owns row r, column jj is shared through LDS at step jj.  On return blk
+// holds L below the diagonal and D on it; dval / dinv the pivots and their inverses.
+__device__ __forceinline__ void diag_factor_lds(double *blk, double *dval, double *dinv, double (*colv)[kNS], int w,
+                                                int col0, int *info, bool clear)
+{
+    const int lane = threadIdx.x & 63;
+    const int r = lane & (kNS - 1);
+    double a[kNS];
+#pragma unroll
+    for (int kk = 0; kk < kNS; ++kk) a[kk] = blk[kk * kNS + r];
+    int bad = 0;
+#pragma unroll
+    for (int jj = 0; jj < kNS; ++jj) {
+        double *col = colv[jj & 1];
+        col[r] = a[jj];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        const double djj = col[jj];
+        if (jj < w && bad == 0 && (djj == 0.0 || !isfinite(djj))) bad = col0 + jj + 1;
+        const double l = a[jj] / djj;
+#pragma unroll
+        for (int kk = jj + 1; kk < kNS; ++kk) a[kk] -= l * col[kk];
+        if (r > jj) a[jj] = l;
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    if (lane == 0 && info && (clear || (bad != 0 && *info == 0))) *info = bad;
+    dval[r] = a[r];
+    dinv[r] = 1.0 / a[r];
+#pragma unroll
+    for (int kk = 0; kk < kNS; ++kk) blk[kk * kNS + r] = kk <= r ? a[kk] : 0.0;
+}
+
+// Panel of 64 columns at c0 with every panel row below the first diagonal block owned by one
+// thread for the whole kernel (bands with bw <= kPT - 32): the thread loads its row's 64 entries
+// once (in two halves, each just before use), solves the first sub-panel's part (TRSM), applies the first sub-panel's
+// rank-32 update to the second part and solves that (TRSM) in registers; the two diagonal
+// blocks go through LDS.  A row's entries are read once and written once.
+template <typename T>
+__global__ __launch_bounds__(kPT) void k_band_panel_rows(const BandDesc *descs, int c0)
+{
+    __shared__ double blk1[kNS * kNS], blk2[kNS * kNS];
+    __shared__ double dval1[kNS], dinv1[kNS], dval2[kNS], dinv2[kNS];
+    __shared__ double colv[2][kNS];
+    __shared__ double lnext[kNS][kNS + 1];  // L[c0+32+j][c0+k] of the first sub-panel
+    const BandDesc &d = descs[blockIdx.x];
+    const int n = d.n, bw = d.bw;
+    if (c0 >= n) return;
+    T *fac = (T *)d.fac;
+    const unsigned ldu = (unsigned)bw + 1;
+    constexpr unsigned es = sizeof(T);
+    const __amdgpu_buffer_rsrc_t rs = band_rsrc(fac, (unsigned)((size_t)n * ldu * es));
+    const int tid = threadIdx.x, wave = tid >> 6;
+    const int w = min(kNB, n - c0), w1 = min(kNS, w), w2 = w - w1;
+    const int nrr = min(n - c0, kNB + bw);  // panel rows (relative to c0)
+    const int r = kNS + tid;                // this thread's row
+    const bool has_row = r < nrr;
+    constexpr int kQ = kNS * kNS / kPT;
+    // loads: the first diagonal block, and this thread's row (columns 0..63 of the panel)
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        const int idx = tid + kPT * q, kk = idx / kNS, rr = idx % kNS;
+        const bool ok = rr >= kk && rr < w1 && rr - kk <= bw;
+        const double v = (double)load_if<T>(ok, rs, ((unsigned)(c0 + kk) * ldu + (unsigned)(rr - kk)) * es);
+        blk1[idx] = (rr >= w1 && rr == kk) ? 1.0 : v;
+    }
+    __syncthreads();
+    if (wave == 0) diag_factor_lds(blk1, dval1, dinv1, colv, w1, c0, d.info, c0 == 0);
+    __syncthreads();
+    // the row is loaded in two halves, each just before it is needed, so that no more than 64
+    // of its values are in registers at once (and none in the factoring wave during a factor)
+    double x[kNB];
+#pragma unroll
+    for (int k = 0; k < kNS; ++k) {
+        const bool ok = has_row && k < w && r - k <= bw;
+        x[k] = (double)load_if<T>(ok, rs, ((unsigned)(c0 + k) * ldu + (unsigned)(r - k)) * es);
+    }
+    // first diagonal block back to the band
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        const int idx = tid + kPT * q, kk = idx / kNS, rr = idx % kNS;
+        const bool ok = rr >= kk && rr < w1 && rr - kk <= bw;
+        store_if<T>(ok, rs, ((unsigned)(c0 + kk) * ldu + (unsigned)(rr - kk)) * es, (T)blk1[idx]);
+    }
+    // TRSM of the first sub-panel: L[r][0:32] = A[r][0:32] (D L_dd^T)^-1
+#pragma unroll
+    for (int kk = 0; kk < kNS; ++kk) {
+        const double tk = x[kk];
+#pragma unroll
+        for (int jj = kk + 1; jj < kNS; ++jj) x[jj] -= tk * blk1[kk * kNS + jj];
+        x[kk] = tk * dinv1[kk];
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    {
+        const int ro = opaque(r);  // store masks recomputed, not kept from the loads
+#pragma unroll
+        for (int k = 0; k < kNS; ++k) {
+            const bool ok = has_row && k < w1 && ro - k <= bw;
+            store_if<T>(ok, rs, ((unsigned)(c0 + k) * ldu + (unsigned)(ro - k)) * es, (T)x[k]);
+        }
+    }
+    if (w2 <= 0) return;  // uniform: no second sub-panel
+#pragma unroll
+    for (int k = kNS; k < kNB; ++k) {
+        const bool ok = has_row && k < w && r >= k && r - k <= bw;
+        x[k] = (double)load_if<T>(ok, rs, ((unsigned)(c0 + k) * ldu + (unsigned)(r - k)) * es);
+    }
+    if (r < kNB) {
+#pragma unroll
+        for (int k = 0; k < kNS; ++k) lnext[r - kNS][k] = x[k];
+    }
+    __syncthreads();
+    // rank-32 update of the second part: x[32+j] -= sum_k L[r][k] d_k L[32+j][k] (j + 32 <= r)
+#pragma unroll
+    for (int k = 0; k < kNS; ++k) x[k] *= dval1[k];
+#pragma unroll
+    for (int j = 0; j < kNS; ++j) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < kNS; ++k) s += x[k] * lnext[j][k];
+        if (kNS + j <= r) x[kNS + j] -= s;
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    // second diagonal block (rows 32..63) through LDS, unit pivots past w2
+    if (r < kNB) {
+        const int rr = r - kNS;
+#pragma unroll
+        for (int kk = 0; kk < kNS; ++kk)
+            blk2[kk * kNS + rr] = rr >= w2 ? (rr == kk ? 1.0 : 0.0) : (kk <= rr ? x[kNS + kk] : 0.0);
+    }
+    __syncthreads();
+    if (wave == 0) diag_factor_lds(blk2, dval2, dinv2, colv, w2, c0 + kNS, d.info, false);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        const int idx = tid + kPT * q, kk = idx / kNS, rr = idx % kNS;
+        const bool ok = rr >= kk && rr < w2 && rr - kk <= bw;
+        store_if<T>(ok, rs, ((unsigned)(c0 + kNS + kk) * ldu + (unsigned)(rr - kk)) * es, (T)blk2[idx]);
+    }
+    if (r < kNB) return;  // rows of the second diagonal block are done
+    // TRSM of the second sub-panel
+#pragma unroll
+    for (int kk = 0; kk < kNS; ++kk) {
+        const double tk = x[kNS + kk];
+#pragma unroll
+        for (int jj = kk + 1; jj < kNS; ++jj) x[kNS + jj] -= tk * blk2[kk * kNS + jj];
+        x[kNS + kk] = tk * dinv2[kk];
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    const int ro = opaque(r);
+#pragma unroll
+    for (int j = 0; j < kNS; ++j) {
+        const int col = kNS + j;
+        const bool ok = has_row && j < w2 && ro - col <= bw;
+        store_if<T>(ok, rs, ((unsigned)(c0 + col) * ldu + (unsigned)(ro - col)) * es, (T)x[col]);
+    }
+}
+
 // lower-triangle block index p -> (bi, bj), bj <= bi
 __device__ __forceinline__ void tri_decode(int p, int &bi, int &bj)
 {
@@ -310,7 +468,10 @@ void run_tiled(hipStream_t s, const BandDesc *d_descs, const std::vector<int> &n
     int active = nbands;
     for (int c0 = 0; c0 < ns[0]; c0 += kNB) {
         while (active > 0 && ns[active - 1] <= c0) --active;
-        hipLaunchKernelGGL(k_band_panel<T>, dim3((unsigned)active), dim3(kPT), 0, s, d_descs, c0);
+        if (maxbw + kNS <= kPT)
+            hipLaunchKernelGGL(k_band_panel_rows<T>, dim3((unsigned)active), dim3(kPT), 0, s, d_descs, c0);
+        else
+            hipLaunchKernelGGL(k_band_panel<T>, dim3((unsigned)active), dim3(kPT), 0, s, d_descs, c0);
         hipLaunchKernelGGL(k_band_trailing<T>, dim3((unsigned)(active * wg_trail)), dim3(64 * kUW), 0, s, d_descs, c0,
                            wg_trail);
     }
