@@ -28,6 +28,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -478,6 +482,73 @@ void run_tiled(hipStream_t s, const BandDesc *d_descs, const std::vector<int> &n
     hipLaunchKernelGGL(k_pivot_stats<T>, dim3((unsigned)nbands), dim3(1024), 0, s, d_descs);
 }
 
+// Captured launch sequences of factor calls.  A call's sequence (about 2 n / 64 launches) depends
+// only on the precision, the device descriptor array, the largest band width and the orders of
+// the bands; it is captured into a hipGraph the second time that key is seen and replayed from
+// then on (the descriptors themselves are uploaded by an ordinary copy before each replay).  At
+// most kGraphs sequences are kept; later keys run eagerly.  KSH_GRAPHS=0 turns capture off.
+struct GraphKey {
+    int prec;
+    const void *descs;
+    int maxbw;
+    std::vector<int> ns;
+    bool operator<(const GraphKey &o) const
+    {
+        if (prec != o.prec) return prec < o.prec;
+        if (descs != o.descs) return descs < o.descs;
+        if (maxbw != o.maxbw) return maxbw < o.maxbw;
+        return ns < o.ns;
+    }
+};
+struct GraphEntry {
+    hipGraphExec_t exec = nullptr;
+    int seen = 0;
+};
+constexpr size_t kGraphs = 64;
+std::mutex g_graph_mu;
+std::map<GraphKey, GraphEntry> g_graphs;
+size_t g_captured = 0;
+
+bool graphs_enabled()
+{
+    const char *e = std::getenv("KSH_GRAPHS");
+    return !(e && std::strcmp(e, "0") == 0);
+}
+
+template <typename T>
+int run_tiled_graph(hipStream_t s, const BandDesc *d_descs, const std::vector<int> &ns, int maxbw, int prec)
+{
+    if (!graphs_enabled()) {
+        run_tiled<T>(s, d_descs, ns, maxbw);
+        return KSH_SUCCESS;
+    }
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    GraphKey key{prec, d_descs, maxbw, ns};
+    auto it = g_graphs.find(key);
+    if (it == g_graphs.end()) {
+        if (g_graphs.size() < 4 * kGraphs) g_graphs.emplace(key, GraphEntry{nullptr, 1});
+        run_tiled<T>(s, d_descs, ns, maxbw);
+        return KSH_SUCCESS;
+    }
+    GraphEntry &en = it->second;
+    if (!en.exec && ++en.seen >= 2 && g_captured < kGraphs) {
+        hipGraph_t g = nullptr;
+        KSH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        run_tiled<T>(s, d_descs, ns, maxbw);
+        KSH_HIP(hipStreamEndCapture(s, &g));
+        const hipError_t e = hipGraphInstantiate(&en.exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(KSH_OTHER_ERROR, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        ++g_captured;
+    }
+    if (en.exec) {
+        KSH_HIP(hipGraphLaunch(en.exec, s));
+    } else {
+        run_tiled<T>(s, d_descs, ns, maxbw);
+    }
+    return KSH_SUCCESS;
+}
+
 }  // namespace
 
 int band_factor_tiled_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHost> &h, int prec)
@@ -500,8 +571,9 @@ int band_factor_tiled_batch(hipStream_t s, void *d_descs, const std::vector<Band
         maxbw = std::max(maxbw, dd[i].bw);
     }
     KSH_HIP(hipMemcpyAsync(d_descs, dd.data(), sizeof(BandDesc) * dd.size(), hipMemcpyHostToDevice, s));
-    if (prec == KSH_FP64) run_tiled<double>(s, (const BandDesc *)d_descs, ns, maxbw);
-    else run_tiled<float>(s, (const BandDesc *)d_descs, ns, maxbw);
+    int rc = prec == KSH_FP64 ? run_tiled_graph<double>(s, (const BandDesc *)d_descs, ns, maxbw, prec)
+                              : run_tiled_graph<float>(s, (const BandDesc *)d_descs, ns, maxbw, prec);
+    if (rc) return rc;
     KSH_HIP(hipGetLastError());
     return KSH_SUCCESS;
 }
