@@ -259,6 +259,15 @@ def main():
                 kernels[k] = {"launches": c, "total_ms": round(ms, 2), "avg_ms": round(ms / c, 4),
                               "achieved": round(r["achieved"], 4), "unit": r["unit"], "frac": round(r["frac"], 5)}
 
+    import ctypes as C
+    tuned = {}
+    for k in ("tangent_tiles", "band_gather"):
+        v, nv = C.c_int64(), C.c_int64()
+        lib.ksh_tune_choice(k.encode(), C.byref(v), C.byref(nv))
+        tuned[k] = {"variant": int(v.value), "variants": int(nv.value),
+                    "shape": (f"{1 << v.value} elements per workgroup" if k == "tangent_tiles"
+                              else f"{128 << v.value} threads per block") if v.value >= 0 else None}
+
     def summary(rs, secs):
         st = int(sum(r["steps_done"] for r in rs))
         return {"steps_per_s": round(st / secs, 3), "converged_steps": st,
@@ -377,6 +386,7 @@ def main():
         "predictor_training": training,
         "predictor_heldout": heldout,
         "roofline": roofline(stats, args.form.startswith("mixed")),
+        "autotuner": tuned,
         "kernels": kernels,
         "cpu_baseline": cpu,
     }

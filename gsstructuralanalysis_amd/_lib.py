@@ -115,6 +115,7 @@ SIGNATURES = {
     "ksh_profile_reset": (C.c_int, []),
     "ksh_profile_query": (C.c_int, [C.c_char_p, _i64p, _dp]),
     "ksh_profile_query_work": (C.c_int, [C.c_char_p, _dp]),
+    "ksh_tune_choice": (C.c_int, [C.c_char_p, _i64p, _i64p]),
     "ksh_run_alm": (C.c_int, [_VP, C.POINTER(AlmOptions), C.POINTER(AlmResult)]),
     "ksh_batch_run_alm": (C.c_int, [C.POINTER(_VP), C.c_int, C.POINTER(AlmOptions), C.POINTER(AlmResult), _dp]),
     "ksh_run_static": (C.c_int, [_VP, C.POINTER(StaticOptions), C.POINTER(StaticResult)]),

@@ -28,6 +28,7 @@
 #include "ksh_mfma.hpp"
 #include "ksh_profile.hpp"
 #include "ksh_shell_math.hpp"
+#include "ksh_tune.hpp"
 
 namespace ksh {
 
@@ -169,18 +170,18 @@ __device__ __forceinline__ int hq_index(int I, int J)
     return r * 15 - (r * (r - 1)) / 2 + (c - r);
 }
 
-// Tangent, stage 2: one wave per element.  The element's H_g (NG x 120 entries) is read once,
+// Tangent, stage 2: one wave per element, E elements per workgroup.  The element's H_g (NG x 120 entries) is read once,
 // coalesced, into the wave's part of LDS.  K_e^{ik} = Psi^T Z^{ik} on 16 x 16 MFMA tiles
 // (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32), inner dimension kk = (g, a) over 5 NG; the
 // B operand Z^{ik}[kk][B] = sum_b H_g[(a,i),(b,k)] Psi_g[b][B] is formed in the lane that feeds
 // it, from H_g (LDS broadcasts) and basis values (tables, L1).  The six tiles i <= k are
 // computed; the tiles i > k are stored as their transposes.
-template <int P, typename T>
-__global__ __launch_bounds__(256) void k_tangent_tiles(const InstDesc *descs, int ninst, int total_el)
+template <int P, typename T, int E>
+__global__ __launch_bounds__(64 * E) void k_tangent_tiles(const InstDesc *descs, int ninst, int total_el)
 {
     constexpr int P1 = P + 1, NB = P1 * P1, NQ = P1, NG = NQ * NQ, NKK = ((5 * NG + 3) / 4) * 4;
     using M = Mfma16<T>;
-    const int eg = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    const int eg = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * E + (threadIdx.x >> 6)));
     if (eg >= total_el) return;
     const InstDesc &d = descs[find_inst(ninst, eg, [&](int m) { return (long long)descs[m].elem0; })];
     const int e = eg - d.elem0, ex = e % d.nx, ey = e / d.nx;
@@ -189,7 +190,7 @@ __global__ __launch_bounds__(256) void k_tangent_tiles(const InstDesc *descs, in
     const bool inA = li < NB;
     const size_t stride = (size_t)d.nel * NG;
     constexpr int kHs = kHq + 1;  // padded row of the LDS copy
-    __shared__ double sH[4][NG][kHs];
+    __shared__ double sH[E][NG][kHs];
     double(*sHw)[kHs] = sH[threadIdx.x >> 6];
     {
         const double *Hb = d.Hg + (size_t)e * NG;
@@ -372,7 +373,7 @@ __global__ void k_vector_gather(const InstDesc *descs, int ninst, int total)
 // ex, ascending); candidates outside the patch or not sharing both points are masked loads (0 from
 // beyond the buffer resource), so every thread issues the same straight-line loads.
 template <int P, typename T>
-__global__ void k_band_gather(const InstDesc *descs, int ninst, long long total)
+__global__ __launch_bounds__(512) void k_band_gather(const InstDesc *descs, int ninst, long long total)
 {
     constexpr int P1 = P + 1;
     long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -541,11 +542,20 @@ int run_tangent(hipStream_t s, InstDesc *d_desc, std::vector<InstDesc> &h, bool 
             work += (double)d.nel * nge * (72.0 * ne + 12.0 * ne * ne);
         }
         ps.set_work(work);
-        const dim3 ge(nblk(t.nel, 4));
-        if (p2 && f64) hipLaunchKernelGGL((k_tangent_tiles<2, double>), ge, b, 0, s, d_desc, n, (int)t.nel);
-        else if (p2) hipLaunchKernelGGL((k_tangent_tiles<2, float>), ge, b, 0, s, d_desc, n, (int)t.nel);
-        else if (f64) hipLaunchKernelGGL((k_tangent_tiles<3, double>), ge, b, 0, s, d_desc, n, (int)t.nel);
-        else hipLaunchKernelGGL((k_tangent_tiles<3, float>), ge, b, 0, s, d_desc, n, (int)t.nel);
+        // variants: 1, 2 or 4 elements (waves) per workgroup; the same arithmetic per element
+        const int tn = (int)t.nel;
+        auto launch_tiles = [&](int v) {
+            const int E = 1 << v;
+            const dim3 ge(nblk(t.nel, E)), be(64 * E);
+#define KSH_TILES(PP, TT, EE) hipLaunchKernelGGL((k_tangent_tiles<PP, TT, EE>), ge, be, 0, s, d_desc, n, tn)
+            if (p2 && f64) { if (E == 1) KSH_TILES(2, double, 1); else if (E == 2) KSH_TILES(2, double, 2); else KSH_TILES(2, double, 4); }
+            else if (p2) { if (E == 1) KSH_TILES(2, float, 1); else if (E == 2) KSH_TILES(2, float, 2); else KSH_TILES(2, float, 4); }
+            else if (f64) { if (E == 1) KSH_TILES(3, double, 1); else if (E == 2) KSH_TILES(3, double, 2); else KSH_TILES(3, double, 4); }
+            else { if (E == 1) KSH_TILES(3, float, 1); else if (E == 2) KSH_TILES(3, float, 2); else KSH_TILES(3, float, 4); }
+#undef KSH_TILES
+        };
+        tune_launch("tangent_tiles", tune_bucket((double)t.nel) * 4 + (p2 ? 0 : 1) * 2 + (f64 ? 1 : 0), 3, 2, s,
+                    launch_tiles);
     }
     KSH_HIP(hipGetLastError());
     ProfScope ps(mass ? "mass_gather" : "band_gather", s);
@@ -555,11 +565,17 @@ int run_tangent(hipStream_t s, InstDesc *d_desc, std::vector<InstDesc> &h, bool 
         for (const auto &d : h) work += (double)d.nel * kKeTile * es + (double)d.nfree * (d.bw + 1.0) * es;
         ps.set_work(work);
     }
-    const dim3 gs(nblk(t.slots, 256));
-    if (p2 && f64) hipLaunchKernelGGL((k_band_gather<2, double>), gs, b, 0, s, d_desc, n, t.slots);
-    else if (p2) hipLaunchKernelGGL((k_band_gather<2, float>), gs, b, 0, s, d_desc, n, t.slots);
-    else if (f64) hipLaunchKernelGGL((k_band_gather<3, double>), gs, b, 0, s, d_desc, n, t.slots);
-    else hipLaunchKernelGGL((k_band_gather<3, float>), gs, b, 0, s, d_desc, n, t.slots);
+    // variants: 128, 256 or 512 threads per block (one band slot per thread either way)
+    auto launch_gather = [&](int v) {
+        const unsigned bs = 128u << v;
+        const dim3 gs(nblk(t.slots, bs)), bb(bs);
+        if (p2 && f64) hipLaunchKernelGGL((k_band_gather<2, double>), gs, bb, 0, s, d_desc, n, t.slots);
+        else if (p2) hipLaunchKernelGGL((k_band_gather<2, float>), gs, bb, 0, s, d_desc, n, t.slots);
+        else if (f64) hipLaunchKernelGGL((k_band_gather<3, double>), gs, bb, 0, s, d_desc, n, t.slots);
+        else hipLaunchKernelGGL((k_band_gather<3, float>), gs, bb, 0, s, d_desc, n, t.slots);
+    };
+    tune_launch(mass ? "mass_gather" : "band_gather", tune_bucket((double)t.slots) * 4 + (p2 ? 0 : 1) * 2 + (f64 ? 1 : 0), 3, 1,
+                s, launch_gather);
     KSH_HIP(hipGetLastError());
     return KSH_SUCCESS;
 }

@@ -153,6 +153,10 @@ int ksh_profile_query(const char *kernel, int64_t *launches, double *total_ms);
  * refine_residual_batch / band_symv: n (2b+1) sizeof(K) + 24 n; band_gather: element tiles
  * read + band written). */
 int ksh_profile_query_work(const char *kernel, double *work);
+/* Launch-shape autotuner: the variant (0-based) the last launch of `kernel` used and the number of
+ * variants ("tangent_tiles": 1, 2, 4 elements per workgroup; "band_gather": 128, 256, 512
+ * threads per block); -1 / 0 if it has not run.  No counterpart in the reference. */
+int ksh_tune_choice(const char *kernel, int64_t *variant, int64_t *variants);
 
 /* ---- solver drivers ---------------------------------------------------------
  * The reference's solver classes (gsALMCrisfield / gsALMRiks, gsStaticNewton +
