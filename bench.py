@@ -261,12 +261,15 @@ def main():
 
     import ctypes as C
     tuned = {}
-    for k in ("tangent_tiles", "band_gather"):
+    shapes = {"tangent_tiles": lambda v: f"{1 << v} elements per workgroup",
+              "band_gather": lambda v: f"{128 << v} threads per block",
+              "band_factor": lambda v: ("plain", "lookahead 8 waves x 1 block", "lookahead 8 waves x 2 blocks")[v]}
+    for k, shape in shapes.items():
         v, nv = C.c_int64(), C.c_int64()
         lib.ksh_tune_choice(k.encode(), C.byref(v), C.byref(nv))
         tuned[k] = {"variant": int(v.value), "variants": int(nv.value),
-                    "shape": (f"{1 << v.value} elements per workgroup" if k == "tangent_tiles"
-                              else f"{128 << v.value} threads per block") if v.value >= 0 else None}
+                    "shape": shape(v.value) if v.value >= 0 else None,
+                    "of": "last call" if k == "band_factor" else "last launch"}
 
     def summary(rs, secs):
         st = int(sum(r["steps_done"] for r in rs))

@@ -39,6 +39,7 @@
 #include "ksh_internal.hpp"
 #include "ksh_mfma.hpp"
 #include "ksh_profile.hpp"
+#include "ksh_tune.hpp"
 
 namespace ksh {
 
@@ -49,6 +50,40 @@ constexpr int kNB = 2 * kNS;  // panel width (rank of one trailing update)
 constexpr int kPT = 512;      // panel kernel threads (TRSM rows per pass)
 constexpr int kUW = 4;        // waves per trailing-update workgroup
 
+// Factor of a 32 x 32 diagonal block held in LDS (blk[kk*32 + r], lower triangle, unit pivots
+// past w) by one wave: lane r owns row r (lanes >= 32 shadow row r - 32) in registers; at step jj
+// column jj (every lane's entry jj) is broadcast with v_readlane, so the wave's FMAs take it as a
+// scalar operand and no LDS round trip sits on the chain of dependent steps.  Entries above the
+// diagonal take updates too; never read.  On return blk holds L below the diagonal and D on it;
+// dval / dinv the pivots and their inverses.  The first bad pivot (zero, not finite, or not
+// finite in the storage precision T) is recorded in *info; `clear` resets it first.
+template <typename T>
+__device__ __forceinline__ void diag_factor(double *blk, double *dval, double *dinv, int w, int col0, int *info, bool clear)
+{
+    const int lane = threadIdx.x & 63;
+    const int r = lane & (kNS - 1);
+    double a[kNS];
+#pragma unroll
+    for (int kk = 0; kk < kNS; ++kk) a[kk] = blk[kk * kNS + r];
+    int bad = 0;
+#pragma unroll
+    for (int jj = 0; jj < kNS; ++jj) {
+        const double cv = a[jj];  // A[r][jj]; lane kk's value is A[kk][jj]
+        const double djj = readlane_f64(cv, jj);
+        if (jj < w && bad == 0 && (djj == 0.0 || !isfinite(djj) || !isfinite((double)(T)djj))) bad = col0 + jj + 1;
+        const double l = cv / djj;
+#pragma unroll
+        for (int kk = jj + 1; kk < kNS; ++kk) a[kk] -= l * readlane_f64(cv, kk);
+        if (r > jj) a[jj] = l;
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    if (lane == 0 && info && (clear || (bad != 0 && *info == 0))) *info = bad;
+    dval[r] = a[r];
+    dinv[r] = 1.0 / a[r];
+#pragma unroll
+    for (int kk = 0; kk < kNS; ++kk) blk[kk * kNS + r] = kk <= r ? a[kk] : 0.0;
+}
+
 // Sub-panel at column cs (width ws = min(32, n - cs)): factor the diagonal block in LDS (wave 0),
 // write it back, and solve the rows below it within the band (row-wise TRSM).  Rows cs+32 ..
 // cs+63 of the result are also left in `lnext` (for the next sub-panel's update).  The panel's
@@ -56,13 +91,13 @@ constexpr int kUW = 4;        // waves per trailing-update workgroup
 // in FP32; the trailing update, almost all of the work, runs in the storage precision).
 template <typename T>
 __device__ __forceinline__ void sub_panel(const BandDesc &d, __amdgpu_buffer_rsrc_t rs, int cs, double *blk, double *dinv,
-                                          double *dval, double (*colv)[kNS], double (*lnext)[kNS + 1])
+                                          double *dval, double (*lnext)[kNS + 1])
 {
     const int n = d.n, bw = d.bw;
     const int w = min(kNS, n - cs), nr = min(n - cs, w + bw);
     const unsigned ldu = (unsigned)bw + 1;
     constexpr unsigned es = sizeof(T);
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, wave = tid >> 6;
     constexpr int kQ = kNS * kNS / kPT;
     // stage the diagonal block (unit pivots pad the last panel)
 #pragma unroll
@@ -73,36 +108,7 @@ __device__ __forceinline__ void sub_panel(const BandDesc &d, __amdgpu_buffer_rsr
         blk[idx] = (r >= w && r == kk) ? 1.0 : v;
     }
     __syncthreads();
-    if (wave == 0) {
-        // lane r owns row cs+r (lanes >= 32 shadow row r-32).  Step jj shares column jj through
-        // LDS (every lane reads the same entries: broadcasts).  Entries above the diagonal take
-        // updates too; never read.
-        const int r = lane & (kNS - 1);
-        double a[kNS];
-#pragma unroll
-        for (int kk = 0; kk < kNS; ++kk) a[kk] = blk[kk * kNS + r];
-        int bad = 0;
-#pragma unroll
-        for (int jj = 0; jj < kNS; ++jj) {
-            double *col = colv[jj & 1];
-            col[r] = a[jj];
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-            const double djj = col[jj];
-            if (jj < w && bad == 0 && (djj == 0.0 || !isfinite(djj) || !isfinite((double)(T)djj))) bad = cs + jj + 1;
-            const double l = a[jj] / djj;
-#pragma unroll
-            for (int kk = jj + 1; kk < kNS; ++kk) a[kk] -= l * col[kk];
-            if (r > jj) a[jj] = l;
-            __builtin_amdgcn_sched_barrier(0);  // keeps the steps' LDS reads from being hoisted together
-        }
-        // first bad pivot of the factorization: the first sub-panel clears the flag
-        if (lane == 0 && d.info && (cs == 0 || (bad != 0 && *d.info == 0))) *d.info = bad;
-        dval[r] = a[r];
-        dinv[r] = 1.0 / a[r];
-#pragma unroll
-        for (int kk = 0; kk < kNS; ++kk) blk[kk * kNS + r] = kk <= r ? a[kk] : 0.0;
-    }
+    if (wave == 0) diag_factor<T>(blk, dval, dinv, w, cs, d.info, cs == 0);
     __syncthreads();
     // write the factored block back (L below the diagonal, D on it)
 #pragma unroll
@@ -142,20 +148,18 @@ __device__ __forceinline__ void sub_panel(const BandDesc &d, __amdgpu_buffer_rsr
 // Panel of 64 columns at c0 as two sub-panels of 32: the second sub-panel's columns take the
 // first one's rank-32 update (A[r][j] -= sum_k L[r][k] d_k L[j][k]) between the two.
 template <typename T>
-__global__ __launch_bounds__(kPT) void k_band_panel(const BandDesc *descs, int c0)
+__device__ __forceinline__ void panel_general(const BandDesc &d, int c0)
 {
     __shared__ double blk[kNS * kNS];     // blk[kk*32 + r] = A / L[cs+r][cs+kk] for r >= kk, zero above
     __shared__ double dinv[kNS], dval[kNS];
-    __shared__ double colv[2][kNS];       // column of the current elimination step (double-buffered)
     __shared__ double lnext[kNS][kNS + 1];  // L[c0+32+j][c0+k] of the first sub-panel
-    const BandDesc &d = descs[blockIdx.x];
     const int n = d.n, bw = d.bw;
     if (c0 >= n) return;
     T *fac = (T *)d.fac;
     const unsigned ldu = (unsigned)bw + 1;
     constexpr unsigned es = sizeof(T);
     const __amdgpu_buffer_rsrc_t rs = band_rsrc(fac, (unsigned)((size_t)n * ldu * es));
-    sub_panel<T>(d, rs, c0, blk, dinv, dval, colv, lnext);
+    sub_panel<T>(d, rs, c0, blk, dinv, dval, lnext);
     const int c1 = c0 + kNS;
     if (c1 >= n) return;  // uniform: the last panel has one sub-panel
     const int w0 = kNS, w1 = min(kNS, n - c1), nr0 = min(n - c0, w0 + bw);
@@ -179,40 +183,7 @@ __global__ __launch_bounds__(kPT) void k_band_panel(const BandDesc *descs, int c
         }
     }
     __syncthreads();  // updated columns visible to the second sub-panel's loads
-    sub_panel<T>(d, rs, c1, blk, dinv, dval, colv, lnext);
-}
-
-// Factor of a 32 x 32 diagonal block held in LDS (blk[kk*32 + r], lower triangle, unit pivots
-// past w) by wave 0: lane r owns row r, column jj is shared through LDS at step jj.  On return blk
-// holds L below the diagonal and D on it; dval / dinv the pivots and their inverses.
-__device__ __forceinline__ void diag_factor_lds(double *blk, double *dval, double *dinv, double (*colv)[kNS], int w,
-                                                int col0, int *info, bool clear)
-{
-    const int lane = threadIdx.x & 63;
-    const int r = lane & (kNS - 1);
-    double a[kNS];
-#pragma unroll
-    for (int kk = 0; kk < kNS; ++kk) a[kk] = blk[kk * kNS + r];
-    int bad = 0;
-#pragma unroll
-    for (int jj = 0; jj < kNS; ++jj) {
-        double *col = colv[jj & 1];
-        col[r] = a[jj];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-        const double djj = col[jj];
-        if (jj < w && bad == 0 && (djj == 0.0 || !isfinite(djj))) bad = col0 + jj + 1;
-        const double l = a[jj] / djj;
-#pragma unroll
-        for (int kk = jj + 1; kk < kNS; ++kk) a[kk] -= l * col[kk];
-        if (r > jj) a[jj] = l;
-        __builtin_amdgcn_sched_barrier(0);
-    }
-    if (lane == 0 && info && (clear || (bad != 0 && *info == 0))) *info = bad;
-    dval[r] = a[r];
-    dinv[r] = 1.0 / a[r];
-#pragma unroll
-    for (int kk = 0; kk < kNS; ++kk) blk[kk * kNS + r] = kk <= r ? a[kk] : 0.0;
+    sub_panel<T>(d, rs, c1, blk, dinv, dval, lnext);
 }
 
 // Panel of 64 columns at c0 with every panel row below the first diagonal block owned by one
@@ -221,13 +192,11 @@ __device__ __forceinline__ void diag_factor_lds(double *blk, double *dval, doubl
 // rank-32 update to the second part and solves that (TRSM) in registers; the two diagonal
 // blocks go through LDS.  A row's entries are read once and written once.
 template <typename T>
-__global__ __launch_bounds__(kPT) void k_band_panel_rows(const BandDesc *descs, int c0)
+__device__ __forceinline__ void panel_rows(const BandDesc &d, int c0)
 {
     __shared__ double blk1[kNS * kNS], blk2[kNS * kNS];
     __shared__ double dval1[kNS], dinv1[kNS], dval2[kNS], dinv2[kNS];
-    __shared__ double colv[2][kNS];
     __shared__ double lnext[kNS][kNS + 1];  // L[c0+32+j][c0+k] of the first sub-panel
-    const BandDesc &d = descs[blockIdx.x];
     const int n = d.n, bw = d.bw;
     if (c0 >= n) return;
     T *fac = (T *)d.fac;
@@ -249,7 +218,7 @@ __global__ __launch_bounds__(kPT) void k_band_panel_rows(const BandDesc *descs, 
         blk1[idx] = (rr >= w1 && rr == kk) ? 1.0 : v;
     }
     __syncthreads();
-    if (wave == 0) diag_factor_lds(blk1, dval1, dinv1, colv, w1, c0, d.info, c0 == 0);
+    if (wave == 0) diag_factor<T>(blk1, dval1, dinv1, w1, c0, d.info, c0 == 0);
     __syncthreads();
     // the row is loaded in two halves, each just before it is needed, so that no more than 64
     // of its values are in registers at once (and none in the factoring wave during a factor)
@@ -313,7 +282,7 @@ __global__ __launch_bounds__(kPT) void k_band_panel_rows(const BandDesc *descs, 
             blk2[kk * kNS + rr] = rr >= w2 ? (rr == kk ? 1.0 : 0.0) : (kk <= rr ? x[kNS + kk] : 0.0);
     }
     __syncthreads();
-    if (wave == 0) diag_factor_lds(blk2, dval2, dinv2, colv, w2, c0 + kNS, d.info, false);
+    if (wave == 0) diag_factor<T>(blk2, dval2, dinv2, w2, c0 + kNS, d.info, false);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kQ; ++q) {
@@ -350,72 +319,149 @@ __device__ __forceinline__ void tri_decode(int p, int &bi, int &bj)
     bj = p - b * (b + 1) / 2;
 }
 
-// A[i][j] -= sum_k L[i][k] d_k L[j][k] over the window rows/columns [w, nr) of the frame at c0
-template <typename T>
-__global__ __launch_bounds__(64 * kUW) void k_band_trailing(const BandDesc *descs, int c0, int wg_per_band)
+// Parts of a step's trailing update (32 x 32 blocks (bi, bj), bj <= bi, of the window):
+// kAll every block; kNext block columns 0 and 1 (the next panel's columns); kRest the others.
+enum TrailPart { kAll = 0, kNext = 1, kRest = 2 };
+
+// Blocks p[0] < p[1] < ... of the given part (wave-uniform), computed together (independent
+// accumulators, the loads of all of them in flight at once): A[i][j] -= sum_k L[i][k] d_k L[j][k]
+// over the window rows/columns [w, nr) of the frame at c0.  Indices past the part's block count
+// are computed on block p[0]'s operands with their stores masked off.  Each block's MFMA sequence
+// (k ascending) is the same for every NBLK, so the result does not depend on it.
+template <typename T, int NBLK>
+__device__ __forceinline__ void trailing_blocks(const BandDesc &d, int c0, int part, const int (&p)[NBLK])
 {
     using M = Mfma16<T>;
     using acc_t = typename M::acc_t;
-    const BandDesc &d = descs[blockIdx.x / wg_per_band];
     const int n = d.n, bw = d.bw;
     if (c0 >= n) return;
     const int w = min(kNB, n - c0), nr = min(n - c0, w + bw);
-    const int nbk = (nr - w + 31) / 32, nblock = nbk * (nbk + 1) / 2;
-    const int p = __builtin_amdgcn_readfirstlane((int)((blockIdx.x % wg_per_band) * kUW + (threadIdx.x >> 6)));
-    if (p >= nblock) return;
+    const int nbk = (nr - w + 31) / 32;
+    const int nblock = part == kAll ? nbk * (nbk + 1) / 2 : part == kNext ? min(nbk, 2) * nbk - (nbk >= 2 ? 1 : 0)
+                                                                          : (nbk > 2 ? (nbk - 2) * (nbk - 1) / 2 : 0);
+    if (p[0] >= nblock) return;
     T *fac = (T *)d.fac;
     const unsigned ldu = (unsigned)bw + 1;
     constexpr unsigned es = sizeof(T);
     const __amdgpu_buffer_rsrc_t rs = band_rsrc(fac, (unsigned)((size_t)n * ldu * es));
     const int lane = threadIdx.x & 63;
-    int bi, bj;
-    tri_decode(p, bi, bj);
     const int li = lane & 15, lk = lane >> 4;
-    const bool diag = bi == bj;
+    bool valid[NBLK], diag[NBLK];
+    int ri0[NBLK], rj0[NBLK];
     // accumulator rows = j (A's column), columns = i (A's row): lanes run along a column
-    acc_t cur[2][2];
-    unsigned off[2][2][4];
+    acc_t cur[NBLK][2][2];
+    unsigned off[NBLK][2][2][4];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int i = w + 32 * bi + 16 * a + li, j = w + 32 * bj + 16 * b + M::row(lane, v);
-                const bool ok = i < nr && i >= j;
-                off[a][b][v] = ok ? ((unsigned)c0 * ldu + (unsigned)j * (ldu - 1) + (unsigned)i) * es : kMaskedOff;
-                cur[a][b][v] = load_vs<T>(rs, off[a][b][v], 0);
+    for (int s = 0; s < NBLK; ++s) {
+        valid[s] = p[s] < nblock;
+        const int q = valid[s] ? p[s] : p[0];
+        int bi, bj;
+        if (part == kNext) {
+            bj = q < nbk ? 0 : 1;
+            bi = bj == 0 ? q : q - nbk + 1;
+        } else {
+            tri_decode(q, bi, bj);
+            if (part == kRest) {
+                bi += 2;
+                bj += 2;
             }
+        }
+        diag[s] = bi == bj;
+        ri0[s] = w + 32 * bi + li;
+        rj0[s] = w + 32 * bj + li;
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int i = w + 32 * bi + 16 * a + li, j = w + 32 * bj + 16 * b + M::row(lane, v);
+                    const bool ok = valid[s] && i < nr && i >= j;
+                    off[s][a][b][v] = ok ? ((unsigned)c0 * ldu + (unsigned)j * (ldu - 1) + (unsigned)i) * es : kMaskedOff;
+                    cur[s][a][b][v] = load_vs<T>(rs, off[s][a][b][v], 0);
+                }
+    }
     // panel operands: rows rj (block column bj) and ri (block row bi), k = 4 s + lk, in chunks
-    const int ri0 = w + 32 * bi + li, rj0 = w + 32 * bj + li;
-    constexpr int kCh = 8;
+    constexpr int kCh = 8 / NBLK;
 #pragma unroll
     for (int c = 0; c < kNB / 4 / kCh; ++c) {
-        T A0[kCh], A1[kCh], B0[kCh], B1[kCh];
+        T A0[NBLK][kCh], A1[NBLK][kCh], B0[NBLK][kCh], B1[NBLK][kCh];
 #pragma unroll
         for (int u = 0; u < kCh; ++u) {
             const int k = 4 * (kCh * c + u) + lk;
             const unsigned col = ((unsigned)c0 * ldu + (unsigned)k * (ldu - 1)) * es;  // + row * es
             const T nd = -load_if<T>(k < w, rs, (unsigned)(c0 + k) * ldu * es);
-            A0[u] = nd * load_if<T>(k < w && rj0 < nr && rj0 - k <= bw, rs, col + (unsigned)rj0 * es);
-            A1[u] = nd * load_if<T>(k < w && rj0 + 16 < nr && rj0 + 16 - k <= bw, rs, col + (unsigned)(rj0 + 16) * es);
-            B0[u] = load_if<T>(k < w && ri0 < nr && ri0 - k <= bw, rs, col + (unsigned)ri0 * es);
-            B1[u] = load_if<T>(k < w && ri0 + 16 < nr && ri0 + 16 - k <= bw, rs, col + (unsigned)(ri0 + 16) * es);
+#pragma unroll
+            for (int s = 0; s < NBLK; ++s) {
+                const int rj = rj0[s], ri = ri0[s];
+                A0[s][u] = nd * load_if<T>(k < w && rj < nr && rj - k <= bw, rs, col + (unsigned)rj * es);
+                A1[s][u] = nd * load_if<T>(k < w && rj + 16 < nr && rj + 16 - k <= bw, rs, col + (unsigned)(rj + 16) * es);
+                B0[s][u] = load_if<T>(k < w && ri < nr && ri - k <= bw, rs, col + (unsigned)ri * es);
+                B1[s][u] = load_if<T>(k < w && ri + 16 < nr && ri + 16 - k <= bw, rs, col + (unsigned)(ri + 16) * es);
+            }
         }
 #pragma unroll
-        for (int u = 0; u < kCh; ++u) {
-            cur[0][0] = M::run(A0[u], B0[u], cur[0][0]);
-            cur[1][0] = M::run(A0[u], B1[u], cur[1][0]);
-            cur[1][1] = M::run(A1[u], B1[u], cur[1][1]);
-            if (!diag) cur[0][1] = M::run(A1[u], B0[u], cur[0][1]);
-        }
+        for (int u = 0; u < kCh; ++u)
+#pragma unroll
+            for (int s = 0; s < NBLK; ++s) {
+                cur[s][0][0] = M::run(A0[s][u], B0[s][u], cur[s][0][0]);
+                cur[s][1][0] = M::run(A0[s][u], B1[s][u], cur[s][1][0]);
+                cur[s][1][1] = M::run(A1[s][u], B1[s][u], cur[s][1][1]);
+                if (!diag[s]) cur[s][0][1] = M::run(A1[s][u], B0[s][u], cur[s][0][1]);
+            }
     }
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int s = 0; s < NBLK; ++s)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) store_vs<T>(rs, off[a][b][v], 0, cur[a][b][v]);
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) store_vs<T>(rs, off[s][a][b][v], 0, cur[s][a][b][v]);
+}
+
+// Panel step at c0 of every band in the grid (one workgroup per band).  kRows: the register-resident
+// form (band widths up to kPT - 32).
+template <typename T, bool kRows>
+__global__ __launch_bounds__(kPT) void k_band_panel(const BandDesc *descs, int c0)
+{
+    if (kRows)
+        panel_rows<T>(descs[blockIdx.x], c0);
+    else
+        panel_general<T>(descs[blockIdx.x], c0);
+}
+
+// One part of the trailing update at c0, kUW blocks (one per wave) per workgroup
+template <typename T>
+__global__ __launch_bounds__(64 * kUW) void k_band_trailing(const BandDesc *descs, int c0, int wg_per_band, int part)
+{
+    const int p[1] = {__builtin_amdgcn_readfirstlane((int)((blockIdx.x % wg_per_band) * kUW + (threadIdx.x >> 6)))};
+    trailing_blocks<T, 1>(descs[blockIdx.x / wg_per_band], c0, part, p);
+}
+
+// Lookahead step: workgroups [0, npanel) factor the panel at c0 + NB of their band while the others
+// apply the rest of the trailing update at c0 (block columns 2 and up): AW waves of each of these
+// workgroups take NBLK blocks each, the other waves leave at once.  The two parts touch disjoint
+// columns: the panel reads and writes columns [c0 + NB, c0 + 2 NB), already updated by the kNext
+// part; the rest of the update reads the panel at c0 and writes columns from c0 + 2 NB on.
+template <typename T, bool kRows, int AW, int NBLK>
+__global__ __launch_bounds__(kPT) void k_band_step(const BandDesc *descs, int c0, int npanel, int wg_per_band)
+{
+    const int b = (int)blockIdx.x;
+    if (b < npanel) {
+        if (kRows)
+            panel_rows<T>(descs[b], c0 + kNB);
+        else
+            panel_general<T>(descs[b], c0 + kNB);
+        return;
+    }
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (wave >= AW) return;
+    const int q = b - npanel, local = q % wg_per_band;
+    int p[NBLK];
+#pragma unroll
+    for (int s = 0; s < NBLK; ++s) p[s] = (local * NBLK + s) * AW + wave;
+    trailing_blocks<T, NBLK>(descs[q / wg_per_band], c0, kRest, p);
 }
 
 // negatives, min d, max |d|, min |d| of each factored diagonal (one workgroup of 1024 threads per band)
@@ -463,23 +509,79 @@ __global__ __launch_bounds__(1024) void k_pivot_stats(const BandDesc *descs)
     }
 }
 
-template <typename T>
-void run_tiled(hipStream_t s, const BandDesc *d_descs, const std::vector<int> &ns, int maxbw)
+// Factor schedules (every one gives the same result bit for bit):
+//   0 plain: panel, then the whole trailing update, per step;
+//   1, 2 lookahead: per step the next panel's columns first (kNext), then the next panel together
+//   with the rest of the update (k_band_step) with AW waves x NBLK blocks per workgroup:
+//   1: 8 x 1, 2: 8 x 2 (more blocks in flight per wave).
+// The lookahead takes the rest of the update off the panel's critical path but runs it at the
+// panel kernel's register budget (2 waves per SIMD); which one is faster depends on the number
+// and shape of the bands, so the autotuner picks per shape bucket (KSH_LOOKAHEAD=k forces k).
+constexpr int kSchedules = 3;
+
+int schedule_forced()
+{
+    const char *e = std::getenv("KSH_LOOKAHEAD");
+    if (!e || !*e) return -1;
+    const int v = std::atoi(e);
+    return v >= 0 && v < kSchedules ? v : -1;
+}
+
+template <typename T, bool kRows, int AW, int NBLK>
+void run_lookahead(hipStream_t s, const BandDesc *d_descs, const std::vector<int> &ns, int maxbw)
+{
+    const int nbands = (int)ns.size();
+    const int nbk = (maxbw + 31) / 32;
+    auto active_at = [&](int c0) {
+        int a = nbands;
+        while (a > 0 && ns[a - 1] <= c0) --a;
+        return a;
+    };
+    const int wg1 = std::max(1, ((nbk >= 2 ? 2 * nbk - 1 : nbk) + kUW - 1) / kUW);
+    const int nrest = nbk > 2 ? (nbk - 2) * (nbk - 1) / 2 : 0;
+    const int wg2 = (nrest + AW * NBLK - 1) / (AW * NBLK);
+    hipLaunchKernelGGL((k_band_panel<T, kRows>), dim3((unsigned)nbands), dim3(kPT), 0, s, d_descs, 0);
+    for (int c0 = 0; c0 < ns[0]; c0 += kNB) {
+        const int active = active_at(c0), np = active_at(c0 + kNB);
+        hipLaunchKernelGGL(k_band_trailing<T>, dim3((unsigned)(active * wg1)), dim3(64 * kUW), 0, s, d_descs, c0, wg1,
+                           (int)kNext);
+        const int grid = np + active * wg2;
+        if (grid > 0)
+            hipLaunchKernelGGL((k_band_step<T, kRows, AW, NBLK>), dim3((unsigned)grid), dim3(kPT), 0, s, d_descs, c0, np,
+                               wg2);
+    }
+}
+
+template <typename T, bool kRows>
+void run_tiled_sched(hipStream_t s, const BandDesc *d_descs, const std::vector<int> &ns, int maxbw, int sched)
 {
     // ns: orders of the bands, descending (the device array is in the same order)
     const int nbands = (int)ns.size();
-    const int nbk = (maxbw + 31) / 32, wg_trail = std::max(1, (nbk * (nbk + 1) / 2 + kUW - 1) / kUW);
-    int active = nbands;
-    for (int c0 = 0; c0 < ns[0]; c0 += kNB) {
-        while (active > 0 && ns[active - 1] <= c0) --active;
-        if (maxbw + kNS <= kPT)
-            hipLaunchKernelGGL(k_band_panel_rows<T>, dim3((unsigned)active), dim3(kPT), 0, s, d_descs, c0);
-        else
-            hipLaunchKernelGGL(k_band_panel<T>, dim3((unsigned)active), dim3(kPT), 0, s, d_descs, c0);
-        hipLaunchKernelGGL(k_band_trailing<T>, dim3((unsigned)(active * wg_trail)), dim3(64 * kUW), 0, s, d_descs, c0,
-                           wg_trail);
+    if (sched == 0) {
+        const int nbk = (maxbw + 31) / 32;
+        const int wg = std::max(1, (nbk * (nbk + 1) / 2 + kUW - 1) / kUW);
+        int active = nbands;
+        for (int c0 = 0; c0 < ns[0]; c0 += kNB) {
+            while (active > 0 && ns[active - 1] <= c0) --active;
+            hipLaunchKernelGGL((k_band_panel<T, kRows>), dim3((unsigned)active), dim3(kPT), 0, s, d_descs, c0);
+            hipLaunchKernelGGL(k_band_trailing<T>, dim3((unsigned)(active * wg)), dim3(64 * kUW), 0, s, d_descs, c0, wg,
+                               (int)kAll);
+        }
+    } else if (sched == 1) {
+        run_lookahead<T, kRows, 8, 1>(s, d_descs, ns, maxbw);
+    } else {
+        run_lookahead<T, kRows, 8, 2>(s, d_descs, ns, maxbw);
     }
     hipLaunchKernelGGL(k_pivot_stats<T>, dim3((unsigned)nbands), dim3(1024), 0, s, d_descs);
+}
+
+template <typename T>
+void run_tiled(hipStream_t s, const BandDesc *d_descs, const std::vector<int> &ns, int maxbw, int sched)
+{
+    if (maxbw + kNS <= kPT)
+        run_tiled_sched<T, true>(s, d_descs, ns, maxbw, sched);
+    else
+        run_tiled_sched<T, false>(s, d_descs, ns, maxbw, sched);
 }
 
 // Captured launch sequences of factor calls.  A call's sequence (about 2 n / 64 launches) depends
@@ -489,12 +591,14 @@ void run_tiled(hipStream_t s, const BandDesc *d_descs, const std::vector<int> &n
 // most kGraphs sequences are kept; later keys run eagerly.  KSH_GRAPHS=0 turns capture off.
 struct GraphKey {
     int prec;
+    int sched;
     const void *descs;
     int maxbw;
     std::vector<int> ns;
     bool operator<(const GraphKey &o) const
     {
         if (prec != o.prec) return prec < o.prec;
+        if (sched != o.sched) return sched < o.sched;
         if (descs != o.descs) return descs < o.descs;
         if (maxbw != o.maxbw) return maxbw < o.maxbw;
         return ns < o.ns;
@@ -516,25 +620,25 @@ bool graphs_enabled()
 }
 
 template <typename T>
-int run_tiled_graph(hipStream_t s, const BandDesc *d_descs, const std::vector<int> &ns, int maxbw, int prec)
+int run_tiled_graph(hipStream_t s, const BandDesc *d_descs, const std::vector<int> &ns, int maxbw, int prec, int la)
 {
-    if (!graphs_enabled()) {
-        run_tiled<T>(s, d_descs, ns, maxbw);
+    if (!graphs_enabled() || tune_measuring()) {
+        run_tiled<T>(s, d_descs, ns, maxbw, la);
         return KSH_SUCCESS;
     }
     std::lock_guard<std::mutex> lk(g_graph_mu);
-    GraphKey key{prec, d_descs, maxbw, ns};
+    GraphKey key{prec, la, d_descs, maxbw, ns};
     auto it = g_graphs.find(key);
     if (it == g_graphs.end()) {
         if (g_graphs.size() < 4 * kGraphs) g_graphs.emplace(key, GraphEntry{nullptr, 1});
-        run_tiled<T>(s, d_descs, ns, maxbw);
+        run_tiled<T>(s, d_descs, ns, maxbw, la);
         return KSH_SUCCESS;
     }
     GraphEntry &en = it->second;
     if (!en.exec && ++en.seen >= 2 && g_captured < kGraphs) {
         hipGraph_t g = nullptr;
         KSH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        run_tiled<T>(s, d_descs, ns, maxbw);
+        run_tiled<T>(s, d_descs, ns, maxbw, la);
         KSH_HIP(hipStreamEndCapture(s, &g));
         const hipError_t e = hipGraphInstantiate(&en.exec, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
@@ -544,9 +648,24 @@ int run_tiled_graph(hipStream_t s, const BandDesc *d_descs, const std::vector<in
     if (en.exec) {
         KSH_HIP(hipGraphLaunch(en.exec, s));
     } else {
-        run_tiled<T>(s, d_descs, ns, maxbw);
+        run_tiled<T>(s, d_descs, ns, maxbw, la);
     }
     return KSH_SUCCESS;
+}
+
+// Schedule: forced by KSH_LOOKAHEAD, else the autotuner's choice for the bucket of (precision,
+// largest band width, number of bands, largest order); its timing calls are whole factor calls.
+template <typename T>
+int run_factor(hipStream_t s, const BandDesc *d_descs, const std::vector<int> &ns, int maxbw, int prec)
+{
+    const int forced = schedule_forced();
+    if (forced >= 0) return run_tiled_graph<T>(s, d_descs, ns, maxbw, prec, forced);
+    const long long bucket =
+        ((((long long)prec * 64 + tune_bucket(maxbw)) * 64 + tune_bucket((double)ns.size())) * 64) + tune_bucket(ns[0]);
+    int rc = KSH_SUCCESS;
+    tune_launch("band_factor", bucket, kSchedules, 0, s,
+                [&](int v) { rc = run_tiled_graph<T>(s, d_descs, ns, maxbw, prec, v); });
+    return rc;
 }
 
 }  // namespace
@@ -571,8 +690,8 @@ int band_factor_tiled_batch(hipStream_t s, void *d_descs, const std::vector<Band
         maxbw = std::max(maxbw, dd[i].bw);
     }
     KSH_HIP(hipMemcpyAsync(d_descs, dd.data(), sizeof(BandDesc) * dd.size(), hipMemcpyHostToDevice, s));
-    int rc = prec == KSH_FP64 ? run_tiled_graph<double>(s, (const BandDesc *)d_descs, ns, maxbw, prec)
-                              : run_tiled_graph<float>(s, (const BandDesc *)d_descs, ns, maxbw, prec);
+    int rc = prec == KSH_FP64 ? run_factor<double>(s, (const BandDesc *)d_descs, ns, maxbw, prec)
+                              : run_factor<float>(s, (const BandDesc *)d_descs, ns, maxbw, prec);
     if (rc) return rc;
     KSH_HIP(hipGetLastError());
     return KSH_SUCCESS;

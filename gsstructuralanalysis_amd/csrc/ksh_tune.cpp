@@ -19,7 +19,10 @@ namespace {
 struct Tuning {
     std::vector<float> ms;  // measured time per variant (negative: not measured yet)
     int best = -1;
+    bool warmed = false;    // the bucket's first call (untimed, default variant) is done
 };
+
+thread_local bool t_measuring = false;
 
 std::mutex g_mu;
 std::map<std::pair<std::string, long long>, Tuning> g_tunings;
@@ -40,6 +43,8 @@ int forced_variant()
 
 }  // namespace
 
+bool tune_measuring() { return t_measuring; }
+
 long long tune_bucket(double x) { return x < 1.0 ? 0 : (long long)std::floor(std::log2(x)); }
 
 int tune_launch(const char *kernel, long long bucket, int nvariants, int fallback, hipStream_t s,
@@ -55,6 +60,10 @@ int tune_launch(const char *kernel, long long bucket, int nvariants, int fallbac
         if (t.ms.empty()) t.ms.assign(nvariants, -1.0f);
         if (t.best >= 0) {
             v = t.best;
+        } else if (!t.warmed) {
+            // first call of the bucket: default variant, untimed (it carries the first-use costs
+            // that would otherwise fall on whichever variant is timed first)
+            t.warmed = true;
         } else {
             for (int i = 0; i < nvariants; ++i)
                 if (t.ms[i] < 0.0f) {
@@ -71,7 +80,9 @@ int tune_launch(const char *kernel, long long bucket, int nvariants, int fallbac
         float ms = 0.0f;
         const bool ev = hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess;
         if (ev) (void)hipEventRecord(a, s);
+        t_measuring = true;
         launch(v);
+        t_measuring = false;
         if (ev) {
             (void)hipEventRecord(b, s);
             (void)hipEventSynchronize(b);
