@@ -49,6 +49,7 @@ constexpr int kNS = 32;       // sub-panel width (diagonal block factored by one
 constexpr int kNB = 2 * kNS;  // panel width (rank of one trailing update)
 constexpr int kPT = 512;      // panel kernel threads (TRSM rows per pass)
 constexpr int kUW = 4;        // waves per trailing-update workgroup
+constexpr int kXr = kNS + 1;  // row stride of the panel update's per-wave LDS region
 
 // Factor of a 32 x 32 diagonal block held in LDS (blk[kk*32 + r], lower triangle, unit pivots
 // past w) by one wave: lane r owns row r (lanes >= 32 shadow row r - 32) in registers; at step jj
@@ -78,8 +79,11 @@ __device__ __forceinline__ void diag_factor(double *blk, double *dval, double *d
         __builtin_amdgcn_sched_barrier(0);
     }
     if (lane == 0 && info && (clear || (bad != 0 && *info == 0))) *info = bad;
-    dval[r] = a[r];
-    dinv[r] = 1.0 / a[r];
+    double ar = 0.0;  // a[r] by selects: a lane-dependent index would put a[] in scratch memory
+#pragma unroll
+    for (int kk = 0; kk < kNS; ++kk) ar = kk == r ? a[kk] : ar;
+    dval[r] = ar;
+    dinv[r] = 1.0 / ar;
 #pragma unroll
     for (int kk = 0; kk < kNS; ++kk) blk[kk * kNS + r] = kk <= r ? a[kk] : 0.0;
 }
@@ -197,6 +201,7 @@ __device__ __forceinline__ void panel_rows(const BandDesc &d, int c0)
     __shared__ double blk1[kNS * kNS], blk2[kNS * kNS];
     __shared__ double dval1[kNS], dinv1[kNS], dval2[kNS], dinv2[kNS];
     __shared__ double lnext[kNS][kNS + 1];  // L[c0+32+j][c0+k] of the first sub-panel
+    __shared__ double xr[kPT / 64][64 * kXr];  // per wave: its 64 rows, first L d, then the update
     const int n = d.n, bw = d.bw;
     if (c0 >= n) return;
     T *fac = (T *)d.fac;
@@ -263,16 +268,48 @@ __device__ __forceinline__ void panel_rows(const BandDesc &d, int c0)
         for (int k = 0; k < kNS; ++k) lnext[r - kNS][k] = x[k];
     }
     __syncthreads();
-    // rank-32 update of the second part: x[32+j] -= sum_k L[r][k] d_k L[32+j][k] (j + 32 <= r)
+    // rank-32 update of the second part, x[32+j] -= S[r][j], S[r][j] = sum_k L[r][k] d_k L[32+j][k],
+    // on FP64 MFMA per wave (entries above the diagonal of rows 32..63 are updated too; never
+    // read).  The wave's rows go through its LDS region: row i of the wave holds L[r][k] d_k at
+    // xr[i*kXr + k], then S[r][j] at xr[i*kXr + j].  S^T tiles (16 j x 16 rows): A = L[32+j][k]
+    // from lnext, B = (L d)^T from the region, so that each value is read once per lane (a
+    // loop over rows with lnext broadcast to every lane costs 32 x 32 LDS broadcasts per wave).
+    {
+        double *reg = xr[wave];
+        const int lane = tid & 63, li = lane & 15, lg = lane >> 4;
 #pragma unroll
-    for (int k = 0; k < kNS; ++k) x[k] *= dval1[k];
+        for (int k = 0; k < kNS; ++k) reg[lane * kXr + k] = x[k] * dval1[k];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        double av[2][kNS / 4];
 #pragma unroll
-    for (int j = 0; j < kNS; ++j) {
-        double s = 0.0;
+        for (int jt = 0; jt < 2; ++jt)
 #pragma unroll
-        for (int k = 0; k < kNS; ++k) s += x[k] * lnext[j][k];
-        if (kNS + j <= r) x[kNS + j] -= s;
-        __builtin_amdgcn_sched_barrier(0);
+            for (int s = 0; s < kNS / 4; ++s) av[jt][s] = lnext[16 * jt + li][4 * s + lg];
+        v4d acc[2][4];
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) {
+            acc[0][rt] = v4d{0.0, 0.0, 0.0, 0.0};
+            acc[1][rt] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s = 0; s < kNS / 4; ++s) {
+                const double bv = reg[(16 * rt + li) * kXr + 4 * s + lg];
+                acc[0][rt] = Mfma16<double>::run(av[0][s], bv, acc[0][rt]);
+                acc[1][rt] = Mfma16<double>::run(av[1][s], bv, acc[1][rt]);
+            }
+        }
+        // lane holds S[16 rt + li][16 jt + row(lane, v)]: back to the row owners through the region
+        // (each row's B reads feed its own tiles, so they precede these writes)
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+            for (int jt = 0; jt < 2; ++jt)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) reg[(16 * rt + li) * kXr + 16 * jt + Mfma16<double>::row(lane, v)] = acc[jt][rt][v];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+#pragma unroll
+        for (int j = 0; j < kNS; ++j) x[kNS + j] -= reg[lane * kXr + j];
     }
     // second diagonal block (rows 32..63) through LDS, unit pivots past w2
     if (r < kNB) {
