@@ -51,10 +51,44 @@ struct SolveCfg {
     static constexpr int DQ = kRB * kRB / NT;  // diagonal-block entries per thread
 };
 
+// Zeros around the vector in LDS, so that the row and column indices of the band products need no
+// clamping: forward columns reach back to r0 - bw >= -512, backward rows up to c0 + 64 + 63 +
+// 64 (PC - 1) <= c0 + 575.
+constexpr int kXPadF = 512;
+constexpr int kXPadB = 640;
+
 template <typename T>
 __host__ __device__ inline size_t solve_lds_bytes(int n)
 {
-    return sizeof(double) * ((size_t)((n + 1) & ~1) + SolveCfg<T>::W * kRB) + sizeof(T) * kRB * kSdLd;
+    return sizeof(double) * ((size_t)kXPadF + ((n + 1) & ~1) + kXPadB + SolveCfg<T>::W * kRB) + sizeof(T) * kRB * kSdLd;
+}
+
+// Sums over the wave's 64 lanes of CW per-lane values (CW a power of two up to 64).  Reduce-
+// scatter by halving: at offset 32 the lanes with that bit clear keep the lower half of the
+// values and receive their partner's, the others the upper half; then offset 16 on the
+// remaining half, and so on down to one value, which a butterfly over the remaining offsets
+// completes.  Lane l returns the total of value l / (64 / CW), after CW - 1 + 6 - log2(CW)
+// exchanges instead of 6 CW.
+template <int CW>
+__device__ __forceinline__ double wave_sums(const double (&a)[CW], int lane)
+{
+    double v[CW];
+#pragma unroll
+    for (int q = 0; q < CW; ++q) v[q] = a[q];
+#pragma unroll
+    for (int lvl = 0; (CW >> lvl) > 1; ++lvl) {
+        const int off = 32 >> lvl, half = (CW >> lvl) / 2;
+        const bool up = (lane & off) != 0;
+#pragma unroll
+        for (int q = 0; q < half; ++q) {
+            const double send = up ? v[q] : v[q + half];
+            const double keep = up ? v[q + half] : v[q];
+            v[q] = keep + __shfl_xor(send, off);
+        }
+    }
+#pragma unroll
+    for (int off = 32 / CW; off > 0; off >>= 1) v[0] += __shfl_xor(v[0], off);
+    return v[0];
 }
 
 template <typename T>
@@ -66,8 +100,9 @@ __device__ __forceinline__ void band_solve_lds(const T *__restrict__ fac, int n,
     constexpr int NT = SolveCfg<T>::NT, kWaves = SolveCfg<T>::W, kPF = SolveCfg<T>::PF;
     constexpr int CW = SolveCfg<T>::CW, DQ = SolveCfg<T>::DQ;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    double *xs = (double *)smem;
-    double *sp = xs + ((n + 1) & ~1);        // [kWaves][64] partial sums
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    double *xs = (double *)smem + kXPadF;
+    double *sp = xs + ((n + 1) & ~1) + kXPadB;  // [kWaves][64] partial sums
     T *sd = (T *)(sp + kWaves * kRB);        // [64][65]: sd[jj*65 + ii] = L[r0+ii][r0+jj]
     const int nblk = (n + kRB - 1) / kRB;
     T pf[kPF];
@@ -92,17 +127,27 @@ __device__ __forceinline__ void band_solve_lds(const T *__restrict__ fac, int n,
     };
 
     for (int i = tid; i < n; i += NT) xs[i] = xg[i];
+    for (int i = tid; i < kXPadF; i += NT) xs[i - kXPadF] = 0.0;
+    for (int i = n + tid; i < ((n + 1) & ~1) + kXPadB; i += NT) xs[i] = 0.0;
 
     // ---- forward: L y = b ----
+    // Entry (i, j), i = r0 + lane, j = r0 - bw + wv + W k (one band column per wave and k) is at
+    // element j (bw + 1) + i - j = j bw + r0 + lane: a per-lane part plus a wave-uniform part,
+    // which the scalar unit computes (kMaskedOff for a column outside [0, r0)).  Per-lane masks
+    // are needed only where rows leave the band (i - j > bw: the first columns, lane > wv + W k)
+    // or the matrix (the last block).
     constexpr int kCh = 8;  // prefetch registers are consumed and refilled in chunks of 8
     auto fwd_load = [&](int r0, int c) {
-        const int i = r0 + lane;
+        const bool row_ok = r0 + lane < n;
 #pragma unroll
         for (int u = 0; u < kCh; ++u) {
             const int k = kCh * c + u;
-            const int j = r0 - bw + wave + kWaves * k;
-            const bool ok = j >= 0 && j < r0 && i < n && i - j <= bw;
-            pf[k] = load_if<T>(ok, rs, (unsigned)((j * ldu + (i - j)) * sizeof(T)));
+            const int j = r0 - bw + wv + kWaves * k;
+            const unsigned so = (j >= 0 && j < r0) ? (unsigned)((j * bw + r0) * (int)sizeof(T)) : kMaskedOff;
+            const unsigned off = (unsigned)lane * (unsigned)sizeof(T) + so;
+            // lane > wv + W k (i - j > bw) only happens for W k < 64 (a compile-time condition)
+            const bool ok = kWaves * k < 64 ? row_ok && lane <= wv + kWaves * k : row_ok;
+            pf[k] = load_vs<T>(rs, ok ? off : kMaskedOff, 0);
         }
     };
 #pragma unroll
@@ -117,11 +162,11 @@ __device__ __forceinline__ void band_solve_lds(const T *__restrict__ fac, int n,
         // load at once, which would exceed the register budget)
 #pragma unroll
         for (int c = 0; c < kPF / kCh; ++c) {
+            const double *xw = xs + (r0 - bw + wv);  // column j of k at xw[W k] (zeros before x)
 #pragma unroll
             for (int u = 0; u < kCh; ++u) {
                 const int k = kCh * c + u;
-                const int j = min(max(r0 - bw + wave + kWaves * k, 0), n - 1);
-                acc += (double)pf[k] * xs[j];
+                acc += (double)pf[k] * xw[kWaves * k];
             }
             fwd_load(r0 + kRB, c);  // past the last block every read is masked off
             __builtin_amdgcn_sched_barrier(0);
@@ -148,13 +193,18 @@ __device__ __forceinline__ void band_solve_lds(const T *__restrict__ fac, int n,
 
     // ---- diagonal and backward: L^T x = D^-1 y ----
     constexpr int kPC = SolveCfg<T>::PC;  // prefetched loads per column
+    // column j = c0 + jj, rows j + d, d = (64 - jj) + 64 k + lane: element j (bw + 1) + d, a
+    // wave-uniform part plus lane; lanes at or past lim (d > bw or j + d >= n) are masked
     auto bwd_load = [&](int c0, int c) {
-        const int jj = CW * wave + c, j = c0 + jj;
+        const int jj = CW * wv + c, j = c0 + jj;
+        const bool cok = c0 >= 0 && j < n;
 #pragma unroll
         for (int k = 0; k < kPC; ++k) {
-            const int d = (kRB - jj) + lane + 64 * k;
-            const bool ok = c0 >= 0 && j < n && d <= bw && j + d < n;
-            pf[c * kPC + k] = load_if<T>(ok, rs, (unsigned)((j * ldu + d) * sizeof(T)));
+            const int dl = (kRB - jj) + 64 * k;
+            const unsigned so = cok ? (unsigned)((j * (int)ldu + dl) * (int)sizeof(T)) : kMaskedOff;
+            unsigned off = (unsigned)lane * (unsigned)sizeof(T) + so;
+            const int lim = min(bw - dl + 1, n - j - dl);
+            pf[c * kPC + k] = load_vs<T>(rs, lane < lim ? off : kMaskedOff, 0);
         }
     };
 #pragma unroll
@@ -163,26 +213,18 @@ __device__ __forceinline__ void band_solve_lds(const T *__restrict__ fac, int n,
     for (int b = nblk - 1; b >= 0; --b) {
         const int c0 = b * kRB;
         double a[CW];
+        const double *xb = xs + c0 + kRB + lane;  // row j + d of k at xb[64 k] (zeros after x)
 #pragma unroll
         for (int c = 0; c < CW; ++c) {
-            const int jj = CW * wave + c, j = c0 + jj;
             double acc = 0.0;
 #pragma unroll
-            for (int k = 0; k < kPC; ++k) {
-                const int r = min(j + (kRB - jj) + lane + 64 * k, n - 1);
-                acc += (double)pf[c * kPC + k] * xs[r];
-            }
+            for (int k = 0; k < kPC; ++k) acc += (double)pf[c * kPC + k] * xb[64 * k];
             a[c] = acc;
             bwd_load(c0 - kRB, c);  // before the first block every read is masked off
             __builtin_amdgcn_sched_barrier(0);
         }
-#pragma unroll
-        for (int c = 0; c < CW; ++c)
-            for (int off = 32; off > 0; off >>= 1) a[c] += __shfl_xor(a[c], off);
-        if (lane == 0) {
-#pragma unroll
-            for (int c = 0; c < CW; ++c) sp[CW * wave + c] = a[c];
-        }
+        const double s = wave_sums<CW>(a, lane);
+        if ((lane & (64 / CW - 1)) == 0) sp[CW * wave + lane / (64 / CW)] = s;
         diag_store();
         diag_load(c0 - kRB, true);
         lds_barrier();
@@ -368,7 +410,7 @@ size_t solve_lds(int nmax, int bwmax, size_t bytes_max, int prec)
 {
     const int cols = prec == KSH_FP64 ? SolveCfg<double>::W * SolveCfg<double>::PF : SolveCfg<float>::W * SolveCfg<float>::PF;
     const size_t smem = prec == KSH_FP64 ? solve_lds_bytes<double>(nmax) : solve_lds_bytes<float>(nmax);
-    if (smem > kSolveLdsMax || bwmax > cols || bwmax > 64 * SolveCfg<float>::PC || bytes_max >= 0x80000000ull) return 0;
+    if (smem > kSolveLdsMax || bwmax > cols || bwmax > 64 * SolveCfg<float>::PC || bytes_max >= 0x7F000000ull) return 0;
     return smem;
 }
 
