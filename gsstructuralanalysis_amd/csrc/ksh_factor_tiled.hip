@@ -11,17 +11,19 @@
 //
 // Layout: lower band, column major, fac[j*ld + (i-j)], ld = bw+1, factored in place into unit L
 // and D.  Right-looking in panels of NB = 64 columns.  Panel step at c0:
-//   1. k_band_panel (one workgroup of 512 threads per band), two sub-panels of 32 columns: the
-//      32 x 32 diagonal block is staged in LDS, wave 0 factors it (lane = row; column jj is
-//      shared through LDS) and writes it back, each thread solves rows below it against L_dd D
-//      (row-wise TRSM) straight from the band; between the two sub-panels the second one's
-//      columns take the first one's rank-32 update.  (One workgroup per band: a second one
-//      would read the diagonal block while the first writes it back.)
-//   2. k_band_trailing (per band a workgroup of 4 waves per 4 blocks): the lower triangle of the
-//      bw x bw trailing window is cut into 32 x 32 blocks, one per wave, each a 2 x 2 set of
-//      16 x 16 MFMA tiles (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32) over the NB panel
-//      columns; panel operands -d_k L[j][k] and L[i][k] come from L2, the block is read, updated
-//      and written once.
+//   1. panel (one workgroup of 512 threads per band), two sub-panels of 32 columns: wave 0
+//      factors the 32 x 32 diagonal block (lane = row, column jj broadcast with v_readlane),
+//      each panel row below it is solved against L_dd D (row-wise TRSM); between the two
+//      sub-panels the second one's columns take the first one's rank-32 update (on MFMA in the
+//      register-resident form).  (One workgroup per band: a second one would read the diagonal
+//      block while the first writes it back.)
+//   2. trailing update: the lower triangle of the bw x bw trailing window is cut into 32 x 32
+//      blocks, one per wave, each a 2 x 2 set of 16 x 16 MFMA tiles (v_mfma_f64_16x16x4f64 /
+//      v_mfma_f32_16x16x4f32) over the NB panel columns; panel operands -d_k L[j][k] and
+//      L[i][k] come from L2, the block is read, updated and written once.
+// Plain schedule: k_band_panel, then k_band_trailing over every block, per step.  Lookahead
+// schedule: the blocks of the next panel's columns first (k_band_trailing), then k_band_step,
+// whose first workgroups factor the next panel while the others apply the remaining blocks.
 // Bands are sorted by order, largest first, so a step's grids cover exactly the bands that are
 // still active.  Work per band: about n bw^2 multiply-adds; HBM traffic about 2 n bw^2 / NB
 // entries (the trailing window of a step, read and written).
