@@ -134,7 +134,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 reference timing")
     ap.add_argument("--form", choices=["mixed+learned", "mixed", "batched+learned", "batched"], default="mixed+learned",
                     help="headline form")
-    ap.add_argument("--epochs", type=int, default=150, help="training epochs of the learned predictor")
+    ap.add_argument("--epochs", type=int, default=400, help="training epochs of the learned predictor")
     ap.add_argument("--only", action="store_true",
                     help="time the headline form only (no other forms, no held-out predictor evaluation)")
     ap.add_argument("--also", nargs="*", default=None, metavar="FORM",
