@@ -48,6 +48,10 @@ struct Req {
     int rc = 0;
     bool done = false;
     int waited = 0;  // engine rounds this request has been left pending
+    // the instance waits on its own request: the engine wakes each one separately, instead of
+    // every instance thread queueing for one mutex after a broadcast
+    std::mutex mu;
+    std::condition_variable cv;
 };
 
 struct DevBuf {
@@ -104,11 +108,14 @@ public:
     // instance side --------------------------------------------------------------------------
     int submit(Req &r)
     {
-        std::unique_lock<std::mutex> lk(m_mu);
-        m_pending.push_back(&r);
-        ++m_blocked;
-        if (m_blocked == m_active) m_cv_main.notify_one();  // the engine waits for all of them
-        m_cv_done.wait(lk, [&] { return r.done; });
+        {
+            std::lock_guard<std::mutex> lk(m_mu);
+            m_pending.push_back(&r);
+            ++m_blocked;
+            if (m_blocked == m_active) m_cv_main.notify_one();  // the engine waits for all of them
+        }
+        std::unique_lock<std::mutex> lk(r.mu);
+        r.cv.wait(lk, [&] { return r.done; });
         return r.rc;
     }
     void finished()
@@ -124,24 +131,35 @@ public:
         int first_error = KSH_SUCCESS;
         for (;;) {
             std::vector<Req *> batch;
+            const auto t0 = std::chrono::steady_clock::now();
             {
                 std::unique_lock<std::mutex> lk(m_mu);
                 m_cv_main.wait(lk, [&] { return m_active == 0 || (m_blocked == m_active && !m_pending.empty()); });
                 if (m_active == 0 && m_pending.empty()) break;
                 select(batch);
             }
+            const auto t1 = std::chrono::steady_clock::now();
             int rc = execute(batch);
+            const auto t2 = std::chrono::steady_clock::now();
+            // host time between rounds (instances running their solver code and submitting) and
+            // time of the round itself, in microseconds
+            prof_count("engine_rounds", 1);
+            prof_count("engine_wait_us", (long long)std::chrono::duration_cast<std::chrono::microseconds>(t1 - t0).count());
+            prof_count("engine_round_us", (long long)std::chrono::duration_cast<std::chrono::microseconds>(t2 - t1).count());
             if (rc && !first_error) first_error = rc;
             {
                 std::lock_guard<std::mutex> lk(m_mu);
-                for (Req *r : batch) {
-                    if (rc && r->rc == KSH_SUCCESS) r->rc = rc;
-                    r->done = true;
-                }
                 m_blocked -= (int)batch.size();
                 ++m_rounds;
             }
-            m_cv_done.notify_all();
+            for (Req *r : batch) {
+                // notified under the request's lock: the instance cannot return (and release
+                // its request) before the engine is done with it
+                std::lock_guard<std::mutex> lk(r->mu);
+                if (rc && r->rc == KSH_SUCCESS) r->rc = rc;
+                r->done = true;
+                r->cv.notify_one();
+            }
         }
         return first_error;
     }
@@ -558,7 +576,7 @@ private:
     }
 
     std::mutex m_mu;
-    std::condition_variable m_cv_main, m_cv_done;
+    std::condition_variable m_cv_main;
     int m_active = 0, m_blocked = 0;
     long long m_rounds = 0;
     long long m_refactors = 0, m_refactor_launches = 0;  // batched FP64 refactorizations (matrices, launches)
