@@ -175,7 +175,7 @@ __device__ __forceinline__ int hq_index(int I, int J)
 // (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32), inner dimension kk = (g, a) over 5 NG; the
 // B operand Z^{ik}[kk][B] = sum_b H_g[(a,i),(b,k)] Psi_g[b][B] is formed in the lane that feeds
 // it, from H_g (LDS broadcasts) and basis values (tables, L1).  The six tiles i <= k are
-// computed; the tiles i > k are stored as their transposes.
+// computed, and K_e's lower triangle is stored from them (the tiles i > k as transposes).
 template <int P, typename T, int E>
 __global__ __launch_bounds__(64 * E) void k_tangent_tiles(const InstDesc *descs, int ninst, int total_el)
 {
@@ -231,7 +231,10 @@ __global__ __launch_bounds__(64 * E) void k_tangent_tiles(const InstDesc *descs,
                 acc[t] = M::run(aop, ok ? (T)z : T(0), acc[t]);  // padded steps add exact zeros
             }
     }
-    // element matrix, local dof 3A + i, column-major with leading dimension kKeLd
+    // element matrix, local dof 3A + i, column-major with leading dimension kKeLd: only its lower
+    // triangle (row >= column), which is all the band gather reads (the local dof order is the
+    // global one), so each entry (3A+i, 3B+k) of tile i <= k is stored once, as itself or as its
+    // transpose, whichever lies on or below the diagonal
     T *Kout = (T *)d.Ke + (size_t)e * kKeTile;
     int t = 0;
 #pragma unroll
@@ -241,8 +244,11 @@ __global__ __launch_bounds__(64 * E) void k_tangent_tiles(const InstDesc *descs,
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int row = M::row(lane, v);  // A; the lane's column li is B
-                Kout[(3 * li + k) * kKeLd + 3 * row + i] = acc[t][v];
-                if (k != i) Kout[(3 * row + i) * kKeLd + 3 * li + k] = acc[t][v];
+                const int r1 = 3 * row + i, c1 = 3 * li + k;
+                if (r1 >= c1)
+                    Kout[c1 * kKeLd + r1] = acc[t][v];
+                else if (k != i)
+                    Kout[r1 * kKeLd + c1] = acc[t][v];
             }
 }
 
