@@ -177,6 +177,7 @@ int inertia_fp64(Matrix &K, int64_t *neg, double *dmin)
     tmp.sprec = KSH_FP64;
     tmp.fprec = KSH_FP64;
     tmp.ab = K.fac64;
+    tmp.ab_clean = false;  // fac64 holds the last factor: every slot is written
     if (K.sprec == KSH_FP64) {
         KSH_HIP(hipMemcpyAsync(K.fac64, K.ab, count * sizeof(double), hipMemcpyDeviceToDevice, pb.stream));
     } else if (K.has_u) {
@@ -459,6 +460,7 @@ int ksh_matrix_create(ksh_problem *pb, int prec, ksh_matrix **out)
     // which must not race with this clear
     if (hipMemsetAsync(K->ab, 0, count * eb, pb->stream) != hipSuccess || hipStreamSynchronize(pb->stream) != hipSuccess)
         return bail(fail(KSH_OTHER_ERROR, "memset"));
+    K->ab_clean = true;
     *out = K;
     return KSH_SUCCESS;
 }
@@ -545,6 +547,7 @@ int ksh_matrix_upload(ksh_matrix *K, const double *ab_in)
         KSH_HIP(hipStreamSynchronize(s));
     }
     K->has_u = false;
+    K->ab_clean = false;
     ++K->version;
     return KSH_SUCCESS;
 }
@@ -554,6 +557,7 @@ int ksh_matrix_axpby(ksh_matrix *Y, double alpha, double beta, const ksh_matrix 
     if (!Y || !X) return fail(KSH_OTHER_ERROR, "null argument");
     if (Y->n != X->n || Y->bw != X->bw) return fail(KSH_OTHER_ERROR, "matrix shapes differ");
     int rc;
+    Y->ab_clean = false;
     if ((rc = band_axpby(*Y, alpha, beta, *X))) return rc;
     Y->has_u = false;
     ++Y->version;
@@ -566,6 +570,7 @@ int ksh_matrix_copy(ksh_matrix *dst, const ksh_matrix *src)
     if (dst->n != src->n || dst->bw != src->bw) return fail(KSH_OTHER_ERROR, "matrix shapes differ");
     size_t count = (size_t)dst->n * dst->ld;
     hipStream_t s = dst->pb->stream;
+    dst->ab_clean = false;  // src may belong to another problem, whose unreached slots differ
     if (dst->sprec == src->sprec) {
         KSH_HIP(hipMemcpyAsync(dst->ab, src->ab, count * elem_bytes(dst->sprec), hipMemcpyDeviceToDevice, s));
     } else {

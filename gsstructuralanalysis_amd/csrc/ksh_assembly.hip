@@ -377,7 +377,9 @@ __global__ void k_vector_gather(const InstDesc *descs, int ninst, int total)
 // Deterministic gather of element matrices into the lower band: one thread per band slot sums
 // the (p+1)^2 candidate elements that could hold both control points, in a fixed order (ey, then
 // ex, ascending); candidates outside the patch or not sharing both points are masked loads (0 from
-// beyond the buffer resource), so every thread issues the same straight-line loads.
+// beyond the buffer resource), so every thread issues the same straight-line loads.  About 5 in 6
+// slots of a p = 3 band are reached by no element; when the destination is known to hold 0 there
+// (skip_zero) those threads store nothing, which leaves the same bits as storing the 0 sum.
 template <int P, typename T>
 __global__ __launch_bounds__(512) void k_band_gather(const InstDesc *descs, int ninst, long long total)
 {
@@ -394,6 +396,9 @@ __global__ __launch_bounds__(512) void k_band_gather(const InstDesc *descs, int 
     const int iI = cpI % d.mx, jI = cpI / d.mx, iJ = cpJ % d.mx, jJ = cpJ / d.mx;
     const int ex0 = max(iI, iJ) - P, exhi = min(min(iI, iJ), d.nx - 1);
     const int ey0 = max(jI, jJ) - P, eyhi = min(min(jI, jJ), d.ny - 1);
+    // no candidate element holds both points (ex0 > exhi or ey0 > eyhi: the index ranges cannot meet
+    // since ex0 + P >= exhi >= 0), or the row is past the end: the slot is 0 and already stored as 0
+    if (d.skip_zero && (!row_ok || ex0 > exhi || ey0 > eyhi)) return;
     const __amdgpu_buffer_rsrc_t rs = band_rsrc(d.Ke, (unsigned)((size_t)d.nel * kKeTile * sizeof(T)));
     T v[P1][P1];
 #pragma unroll
@@ -712,8 +717,11 @@ int launch_tangent(Problem &pb, Matrix &K)
     InstDesc d = make_desc(pb);
     d.ab = K.ab;
     d.sprec = K.sprec;
+    d.skip_zero = K.ab_clean ? 1 : 0;
     std::vector<InstDesc> h = {d};
-    return run_tangent(pb.stream, pb.d_desc, h, false, false);
+    const int rc = run_tangent(pb.stream, pb.d_desc, h, false, false);
+    K.ab_clean = rc == KSH_SUCCESS;
+    return rc;
 }
 
 int launch_mass(Problem &pb, Matrix &M)
@@ -721,8 +729,11 @@ int launch_mass(Problem &pb, Matrix &M)
     InstDesc d = make_desc(pb);
     d.ab = M.ab;
     d.sprec = M.sprec;
+    d.skip_zero = M.ab_clean ? 1 : 0;
     std::vector<InstDesc> h = {d};
-    return run_tangent(pb.stream, pb.d_desc, h, false, true);
+    const int rc = run_tangent(pb.stream, pb.d_desc, h, false, true);
+    M.ab_clean = rc == KSH_SUCCESS;
+    return rc;
 }
 
 }  // namespace ksh

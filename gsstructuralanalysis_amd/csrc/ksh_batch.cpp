@@ -270,10 +270,12 @@ private:
                 d.u_free = din + off[i];
                 d.ab = K.ab;
                 d.sprec = K.sprec;
+                d.skip_zero = K.ab_clean ? 1 : 0;
                 h.push_back(d);
             }
             if (!h.empty() && (rc = batch_tangent(m_stream, (InstDesc *)m_desc.p, h))) return rc;
         }
+        for (Req *r : g) r->K->ab_clean = true;
         std::vector<Matrix *> ms;
         for (size_t i = 0; i < g.size(); ++i) {
             Matrix &K = *g[i]->K;

@@ -51,7 +51,7 @@ struct InstDesc {
     int elem0;              // first element of this instance in the launch
     int dof0;               // first free dof
     int full0;              // first full dof
-    int pad_;
+    int skip_zero;          // 1: band slots no element reaches already hold 0 in ab and are not stored
     long long slot0;        // first band slot (n*(bw+1) per instance)
     double t, rho, lambda;
     int mode, sprec;        // residual gather mode (0 sum, 1 ALM, 2 static); band storage precision
@@ -116,6 +116,7 @@ struct Matrix {
     double *u_at = nullptr;        // free vector the matrix was assembled at (FP32 storage)
     bool has_u = false;
     bool factored = false;
+    bool ab_clean = false;         // every band slot that no element reaches holds 0 in ab
     uint64_t version = 0;          // bumped on every change of ab
     uint64_t factored_version = ~0ull;
     int info = 0;                  // 0 or 1-based index of a zero / non-finite pivot

@@ -87,3 +87,23 @@ def test_assembly_is_deterministic(ksh):
     R1 = sp.residual(u, 0.3)
     R2 = sp.residual(u, 0.3)
     assert np.array_equal(K1, K2) and np.array_equal(R1, R2)
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2])
+def test_reassembly_over_written_band(prec, ksh):
+    """A band assembled afresh, re-assembled in place, and re-assembled after upload / axpby left
+    values in slots no element reaches holds the same bits each time; those slots are exactly 0,
+    as in the oracle's band."""
+    pt, sp, o = _pair("roof_p3_n6x3")
+    rng = np.random.default_rng(10)
+    u0, u1 = (rng.standard_normal(o.nfree) * 1e-2 for _ in range(2))
+    ref = sp.tangent(u1, precision=prec).download()
+    unreached = o.tangent_band(u1) == 0.0
+    assert unreached.mean() > 0.3 and np.all(ref[unreached] == 0.0)
+    K = sp.tangent(u0, precision=prec)
+    assert np.array_equal(sp.tangent(u1, out=K).download(), ref)
+    K.upload(np.ones(K.n * (K.bw + 1)))
+    assert np.array_equal(sp.tangent(u1, out=K).download(), ref)
+    K.axpby(1.0, 2.0, sp.tangent(u0, precision=prec))
+    assert np.array_equal(sp.tangent(u1, out=K).download(), ref)
+    assert np.array_equal(sp.tangent(u1, out=K).download(), ref)
