@@ -46,7 +46,8 @@ METRIC = "equilibrium steps/s at ‖R‖/‖F‖≤1e-8: mixed+batched+learned v
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "hbm": 8.0e12}
 KERNELS = ["tangent_gp", "tangent_elem", "band_gather", "residual_elem", "vector_gather", "band_ldlt", "band_solve", "band_symv",
            "band_ldlt_batch_fp64", "band_ldlt_batch_fp32", "band_solve_batch_fp64", "band_solve_batch_fp32",
-           "refine_residual_batch", "fp64_refactor_guard", "fp64_refactor_stall", "band_ldlt_tiled", "band_solve_tiled",
+           "refine_residual_batch", "fp64_refactor_guard", "fp64_refactor_stall", "fp64_refactor_early",
+           "fp64_refactor_early_used", "band_ldlt_tiled", "band_solve_tiled",
            "engine_rounds", "engine_wait_us", "engine_round_us"]
 
 

@@ -157,6 +157,11 @@ void destroy_problem(Problem *pb)
 void destroy_matrix(Matrix *K)
 {
     if (!K) return;
+    if (K->early_done) {  // an early FP64 factor of a batch engine may still be writing fac64
+        (void)hipEventSynchronize(K->early_done);
+        (void)hipEventDestroy(K->early_done);
+        K->early_done = nullptr;
+    }
     void *ptrs[] = {K->ab, K->fac, K->fac64, K->u_at, K->work, K->flag, K->red, K->desc};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -172,6 +177,7 @@ int inertia_fp64(Matrix &K, int64_t *neg, double *dmin)
         K.fac64 = dalloc<double>(count, rc);
         if (!K.fac64) return rc;
     }
+    if (K.early_done) KSH_HIP(hipStreamWaitEvent(pb.stream, K.early_done, 0));
     Matrix tmp = K;
     tmp.prec = KSH_FP64;
     tmp.sprec = KSH_FP64;

@@ -97,11 +97,24 @@ public:
     explicit Engine(int nthreads) : m_active(nthreads) {}
     ~Engine()
     {
+        if (m_early_stream) {
+            (void)hipStreamSynchronize(m_early_stream);
+            (void)hipStreamDestroy(m_early_stream);
+        }
+        if (m_ev_copied) (void)hipEventDestroy(m_ev_copied);
+        if (m_ev_early) (void)hipEventDestroy(m_ev_early);
         if (m_stream) (void)hipStreamDestroy(m_stream);
     }
     int init()
     {
         KSH_HIP(hipStreamCreateWithFlags(&m_stream, hipStreamNonBlocking));
+        const char *e = std::getenv("KSH_EARLY64");
+        m_early_on = !(e && std::strcmp(e, "0") == 0);
+        if (m_early_on) {
+            KSH_HIP(hipStreamCreateWithFlags(&m_early_stream, hipStreamNonBlocking));
+            KSH_HIP(hipEventCreateWithFlags(&m_ev_copied, hipEventDisableTiming));
+            KSH_HIP(hipEventCreateWithFlags(&m_ev_early, hipEventDisableTiming));
+        }
         return KSH_SUCCESS;
     }
 
@@ -369,6 +382,7 @@ private:
     {
         if (ms.empty()) return KSH_SUCCESS;
         int rc;
+        if ((rc = retire_early())) return rc;
         if ((rc = ensure_desc(ms.size()))) return rc;
         for (int fp : {KSH_FP64, KSH_FP32}) {
             std::vector<BandDescHost> h;
@@ -382,6 +396,14 @@ private:
                 grp.push_back(K);
             }
             if (grp.empty()) continue;
+            if (fp == KSH_FP32) {
+                // a solve of this matrix's previous FP32 factor fell back to FP64: so will one of
+                // this factor's, most likely; its FP64 factor runs beside the FP32 one
+                std::vector<Matrix *> early;
+                for (Matrix *K : grp)
+                    if (K->stalled && K->sprec == KSH_FP64) early.push_back(K);
+                if ((rc = launch_early(early))) return rc;
+            }
             if ((rc = band_copy_batch(m_stream, m_cdesc.p, cp))) return rc;
             if ((rc = band_factor_batch(m_stream, m_bdesc.p, h, fp))) return rc;
             std::vector<int> info(grp.size());
@@ -392,6 +414,7 @@ private:
             std::vector<Matrix *> re;
             for (size_t k = 0; k < grp.size(); ++k) {
                 Matrix &K = *grp[k];
+                K.stalled = false;
                 K.factored = true;
                 K.factored_version = K.version;
                 K.use64 = false;
@@ -402,9 +425,112 @@ private:
                 if (fp == KSH_FP32 && (info[k] != 0 || !(K.fstats[3] >= K.guard * K.fstats[2]))) re.push_back(&K);
             }
             prof_count("fp64_refactor_guard", (long long)re.size());
-            if ((rc = refactor64_batch(re))) return rc;
+            if ((rc = refactor64(re))) return rc;
         }
         return KSH_SUCCESS;
+    }
+
+    // FP64 factors of the current values of several matrices (each at most once): the early
+    // factor where one was started for this version, else one batched refactorization.
+    int refactor64(const std::vector<Matrix *> &ms)
+    {
+        std::vector<Matrix *> re;
+        std::vector<std::pair<Matrix *, long>> taken;
+        for (Matrix *K : ms) {
+            if (std::find(re.begin(), re.end(), K) != re.end() ||
+                std::find_if(taken.begin(), taken.end(), [&](const auto &t) { return t.first == K; }) != taken.end())
+                continue;
+            const long k = early_index(K);
+            if (k >= 0) taken.emplace_back(K, k);
+            else re.push_back(K);
+        }
+        if (!taken.empty()) {
+            KSH_HIP(hipEventSynchronize(m_ev_early));
+            KSH_HIP(hipStreamWaitEvent(m_stream, m_ev_early, 0));
+            const size_t n = m_early.size();
+            const int *hi = (const int *)m_hearly.p;
+            const double *hs = (const double *)((const char *)m_hearly.p + n * sizeof(int));
+            for (const auto &t : taken) {  // as refactor64_batch leaves a matrix
+                Matrix &K = *t.first;
+                const long k = t.second;
+                K.use64 = true;
+                K.info = hi[k] == 0 ? 0 : 1;
+                for (int q = 0; q < 4; ++q) K.fstats[q] = hs[4 * k + q];
+                K.has_stats = hi[k] == 0;
+            }
+            prof_count("fp64_refactor_early_used", (long long)taken.size());
+        }
+        return refactor64_batch(re);
+    }
+
+    // Early FP64 factors.  A matrix whose previous FP32 factor needed the FP64 fallback in a
+    // solve is likely to need it again (the stalls follow the conditioning, which changes
+    // slowly along the path), so its FP64 factor is started together with the FP32 one, on a
+    // second stream: the two launch chains, each bound by its panel latency, run at once.  A
+    // solve that stalls (or a guard refactorization) takes that factor instead of refactoring:
+    // the same kernels on the same values, so the same bits, pivot statistics and decisions as
+    // the refactorization it replaces; unused early factors are dropped.
+    // Only FP64 storage (the copy of ab); FP32 storage re-assembles on the problem's element
+    // tiles, which the main stream may be using.  fac64 of these matrices belongs to the second
+    // stream until retire_early (start of the next factor call) or until a stall takes it.
+    int launch_early(const std::vector<Matrix *> &ms)
+    {
+        if (!m_early_on || ms.empty()) return KSH_SUCCESS;
+        int rc;
+        const size_t n = ms.size();
+        const size_t hb = n * (sizeof(int) + 4 * sizeof(double));
+        if (m_ebdesc.bytes < n * kBandDescBytes || m_einfo.bytes < n * sizeof(int) ||
+            m_estats.bytes < n * 4 * sizeof(double) || m_hearly.bytes < hb)
+            KSH_HIP(hipStreamSynchronize(m_early_stream));  // buffers about to be replaced
+        if ((rc = m_ebdesc.ensure(n * kBandDescBytes)) || (rc = m_einfo.ensure(n * sizeof(int))) ||
+            (rc = m_estats.ensure(n * 4 * sizeof(double))) || (rc = m_hearly.ensure(hb)))
+            return rc;
+        std::vector<CopyDesc> cp;
+        std::vector<BandDescHost> h;
+        m_early.clear();
+        for (size_t k = 0; k < n; ++k) {
+            Matrix *K = ms[k];
+            if ((rc = ensure_fac64(*K))) return rc;
+            cp.push_back(CopyDesc{K->ab, K->fac64, (size_t)K->n * K->ld, K->sprec, KSH_FP64, 0});
+            h.push_back(BandDescHost{K->fac64, K->n, K->bw, (int *)m_einfo.p + k, (double *)m_estats.p + 4 * k, nullptr, 1});
+            m_early.push_back(Early{K, K->version});  // the version about to be factored
+        }
+        if ((rc = band_copy_batch(m_stream, m_cdesc.p, cp))) return rc;
+        KSH_HIP(hipEventRecord(m_ev_copied, m_stream));
+        KSH_HIP(hipStreamWaitEvent(m_early_stream, m_ev_copied, 0));
+        if ((rc = band_factor_batch(m_early_stream, m_ebdesc.p, h, KSH_FP64))) return rc;
+        char *hp = (char *)m_hearly.p;
+        KSH_HIP(hipMemcpyAsync(hp, m_einfo.p, n * sizeof(int), hipMemcpyDeviceToHost, m_early_stream));
+        KSH_HIP(hipMemcpyAsync(hp + n * sizeof(int), m_estats.p, n * 4 * sizeof(double), hipMemcpyDeviceToHost,
+                               m_early_stream));
+        KSH_HIP(hipEventRecord(m_ev_early, m_early_stream));
+        for (Matrix *K : ms) {  // the matrix's own copy of the event, for whoever frees or rewrites fac64
+            if (!K->early_done) KSH_HIP(hipEventCreateWithFlags(&K->early_done, hipEventDisableTiming));
+            KSH_HIP(hipEventRecord(K->early_done, m_early_stream));
+        }
+        m_early_pending = true;
+        prof_count("fp64_refactor_early", (long long)n);
+        return KSH_SUCCESS;
+    }
+
+    // The main stream waits for the early factors (their fac64 may be written again) and they
+    // are forgotten.
+    int retire_early()
+    {
+        if (m_early_pending) KSH_HIP(hipStreamWaitEvent(m_stream, m_ev_early, 0));
+        m_early_pending = false;
+        m_early.clear();
+        return KSH_SUCCESS;
+    }
+
+    // Position of K's early factor of its current version, or -1.  Compares addresses only:
+    // the entries are dropped at every factor call, and a matrix can only be freed and another
+    // created at its address after that.
+    long early_index(const Matrix *K) const
+    {
+        for (size_t k = 0; k < m_early.size(); ++k)
+            if (m_early[k].K == K && m_early[k].version == K->factored_version) return (long)k;
+        return -1;
     }
 
     // FP64 refactorization of several matrices whose FP32 factor cannot be trusted, in one
@@ -541,13 +667,17 @@ private:
             for (size_t i = 0; i < g.size(); ++i)
                 if (last[i]) active[i] = 0;
         }
-        // FP64 fallback of stalled refinements: x = K64^-1 b (one batched refactorization)
+        // FP64 fallback of stalled refinements: x = K64^-1 b, from the early FP64 factor where
+        // one was started, else from one batched refactorization
         {
             std::vector<Matrix *> re;
-            for (size_t i = 0; i < g.size(); ++i)
-                if (fallback[i]) re.push_back(g[i]->K);
+            for (size_t i = 0; i < g.size(); ++i) {
+                if (!fallback[i]) continue;
+                g[i]->K->stalled = true;
+                re.push_back(g[i]->K);
+            }
             prof_count("fp64_refactor_stall", (long long)re.size());
-            if ((rc = refactor64_batch(re))) return rc;
+            if ((rc = refactor64(re))) return rc;
             std::vector<BandDescHost> h64;
             for (size_t i = 0; i < g.size(); ++i) {
                 if (!fallback[i]) continue;
@@ -584,6 +714,17 @@ private:
     long long m_refactors = 0, m_refactor_launches = 0;  // batched FP64 refactorizations (matrices, launches)
     std::vector<Req *> m_pending;
     hipStream_t m_stream = nullptr;
+    // early FP64 factors (launch_early)
+    struct Early {
+        const Matrix *K;
+        uint64_t version;
+    };
+    bool m_early_on = false, m_early_pending = false;
+    hipStream_t m_early_stream = nullptr;
+    hipEvent_t m_ev_copied = nullptr, m_ev_early = nullptr;
+    std::vector<Early> m_early;
+    DevBuf m_ebdesc, m_einfo, m_estats;
+    HostBuf m_hearly;
     DevBuf m_din[3], m_dout, m_dr, m_desc, m_bdesc, m_cdesc, m_rdesc, m_stats, m_info, m_norms, m_part;
     HostBuf m_hin[3], m_hout;
     size_t m_tot = 0;

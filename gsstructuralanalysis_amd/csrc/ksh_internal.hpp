@@ -117,6 +117,8 @@ struct Matrix {
     bool has_u = false;
     bool factored = false;
     bool ab_clean = false;         // every band slot that no element reaches holds 0 in ab
+    bool stalled = false;          // a solve of the current FP32 factor fell back to FP64 (batch engine)
+    hipEvent_t early_done = nullptr;  // recorded after the last early FP64 factor into fac64 (batch engine)
     uint64_t version = 0;          // bumped on every change of ab
     uint64_t factored_version = ~0ull;
     int info = 0;                  // 0 or 1-based index of a zero / non-finite pivot

@@ -70,3 +70,46 @@ def test_batch_with_failures_and_uneven_work(ksh):
         assert b["steps_done"] == s["steps_done"]
         assert b["halvings"] == s["halvings"]
         assert np.allclose(b["lambda"], s["lambda"], rtol=1e-12, atol=0)
+
+
+def _count(name):
+    import ctypes as C
+    from gsstructuralanalysis_amd import _lib as L
+    cnt, ms = C.c_int64(), C.c_double()
+    L.load().ksh_profile_query(name.encode(), C.byref(cnt), C.byref(ms))
+    return cnt.value
+
+
+def test_early_fp64_factors_change_nothing(ksh, monkeypatch):
+    """The engine starts the FP64 factor of a matrix whose previous FP32 factor needed the FP64
+    fallback on a second stream, before any solve asks for it.  With one refinement sweep and an
+    acceptance of 1e-13 nearly every FP32 solve falls back, so most fallbacks take an early
+    factor: the batch with early factors, the batch without them (KSH_EARLY64=0) and the single
+    runs give the same path."""
+    from gsstructuralanalysis_amd import _lib as L
+    from gsstructuralanalysis_amd.problem import ShellProblem, run_alm_batch
+    pts = [g.imperfect_roof_instance(i, sizes=(6, 8, 10)) for i in range(6)]
+    print(f"[case] early FP64 factors, batch of {len(pts)} imperfect roofs, forced fallbacks", flush=True)
+    extra = dict(precision=1, max_refine=1, refine_tol=1e-12, refine_accept=1e-13)
+    sps = [ShellProblem(pt) for pt in pts]
+    lib = L.load()
+    lib.ksh_profile_reset()
+    lib.ksh_profile_enable(1)
+    try:
+        early, _ = run_alm_batch(sps, 12, 4e-3, keep_U=True, **KW, **extra)
+    finally:
+        lib.ksh_profile_enable(0)
+    used, stalls = _count("fp64_refactor_early_used"), _count("fp64_refactor_stall")
+    print(f"[case] fallbacks {stalls}, taken from early factors {used}", flush=True)
+    assert used > stalls // 2
+    monkeypatch.setenv("KSH_EARLY64", "0")
+    plain, _ = run_alm_batch(sps, 12, 4e-3, keep_U=True, **KW, **extra)
+    for sp, a, b in zip(sps, early, plain):
+        s = sp.run_alm(12, 4e-3, keep_U=True, **KW, **extra)
+        for r in (a, b):
+            assert r["steps_done"] == s["steps_done"] == 12
+            assert np.array_equal(r["negatives"], s["negatives"])
+            assert np.array_equal(r["iterations"], s["iterations"])
+            assert np.abs(r["U"] - s["U"]).max() <= 1e-9 * np.abs(s["U"]).max()
+            assert np.abs(r["lambda"] - s["lambda"]).max() <= 1e-9 * np.abs(s["lambda"]).max()
+        assert np.array_equal(a["U"], b["U"]) and np.array_equal(a["lambda"], b["lambda"])
