@@ -498,7 +498,7 @@ private:
         if ((rc = band_copy_batch(m_stream, m_cdesc.p, cp))) return rc;
         KSH_HIP(hipEventRecord(m_ev_copied, m_stream));
         KSH_HIP(hipStreamWaitEvent(m_early_stream, m_ev_copied, 0));
-        if ((rc = band_factor_batch(m_early_stream, m_ebdesc.p, h, KSH_FP64))) return rc;
+        if ((rc = band_factor_batch(m_early_stream, m_ebdesc.p, h, KSH_FP64, "band_ldlt_early_fp64"))) return rc;
         char *hp = (char *)m_hearly.p;
         KSH_HIP(hipMemcpyAsync(hp, m_einfo.p, n * sizeof(int), hipMemcpyDeviceToHost, m_early_stream));
         KSH_HIP(hipMemcpyAsync(hp + n * sizeof(int), m_estats.p, n * 4 * sizeof(double), hipMemcpyDeviceToHost,

@@ -437,7 +437,7 @@ int band_factor(Matrix &K, void *fac, int prec, double *stats_out)
     return KSH_SUCCESS;
 }
 
-int band_factor_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHost> &h, int prec)
+int band_factor_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHost> &h, int prec, const char *prof_name)
 {
     if (h.empty()) return KSH_SUCCESS;
     int maxbw = 0, maxn = 0;
@@ -448,7 +448,7 @@ int band_factor_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHo
         maxbw = std::max(maxbw, h[i].bw);
         maxn = std::max(maxn, h[i].n);
     }
-    ProfScope ps(prec == KSH_FP64 ? "band_ldlt_batch_fp64" : "band_ldlt_batch_fp32", s);
+    ProfScope ps(prof_name ? prof_name : prec == KSH_FP64 ? "band_ldlt_batch_fp64" : "band_ldlt_batch_fp32", s);
     {
         double work = 0.0;  // n (b^2 + 3b) flop per matrix
         for (const auto &x : h)

@@ -163,7 +163,9 @@ struct BandDesc {
     int active;      // 0: skip this matrix
     int pad_;
 };
-int band_factor_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHost> &h, int prec);
+// prof_name: name of the launch in the HIP-event profile (default band_ldlt_batch_fp64 / _fp32)
+int band_factor_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHost> &h, int prec,
+                      const char *prof_name = nullptr);
 // factor of a batch of bands over the whole GPU (ksh_factor_tiled.hip): panel steps of two
 // launches covering every band still active; d_descs holds h.size() device BandDescs
 int band_factor_tiled_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHost> &h, int prec);
