@@ -487,18 +487,21 @@ private:
             return rc;
         std::vector<CopyDesc> cp;
         std::vector<BandDescHost> h;
-        m_early.clear();
+        std::vector<Early> entries;  // published only once everything is queued
         for (size_t k = 0; k < n; ++k) {
             Matrix *K = ms[k];
             if ((rc = ensure_fac64(*K))) return rc;
             cp.push_back(CopyDesc{K->ab, K->fac64, (size_t)K->n * K->ld, K->sprec, KSH_FP64, 0});
             h.push_back(BandDescHost{K->fac64, K->n, K->bw, (int *)m_einfo.p + k, (double *)m_estats.p + 4 * k, nullptr, 1});
-            m_early.push_back(Early{K, K->version});  // the version about to be factored
+            entries.push_back(Early{K, K->version});  // the version about to be factored
         }
         if ((rc = band_copy_batch(m_stream, m_cdesc.p, cp))) return rc;
         KSH_HIP(hipEventRecord(m_ev_copied, m_stream));
         KSH_HIP(hipStreamWaitEvent(m_early_stream, m_ev_copied, 0));
-        if ((rc = band_factor_batch(m_early_stream, m_ebdesc.p, h, KSH_FP64, "band_ldlt_early_fp64"))) return rc;
+        if ((rc = band_factor_batch(m_early_stream, m_ebdesc.p, h, KSH_FP64, "band_ldlt_early_fp64"))) {
+            (void)hipStreamSynchronize(m_early_stream);  // nothing of a failed launch left in flight
+            return rc;
+        }
         char *hp = (char *)m_hearly.p;
         KSH_HIP(hipMemcpyAsync(hp, m_einfo.p, n * sizeof(int), hipMemcpyDeviceToHost, m_early_stream));
         KSH_HIP(hipMemcpyAsync(hp + n * sizeof(int), m_estats.p, n * 4 * sizeof(double), hipMemcpyDeviceToHost,
@@ -508,6 +511,7 @@ private:
             if (!K->early_done) KSH_HIP(hipEventCreateWithFlags(&K->early_done, hipEventDisableTiming));
             KSH_HIP(hipEventRecord(K->early_done, m_early_stream));
         }
+        m_early.swap(entries);
         m_early_pending = true;
         prof_count("fp64_refactor_early", (long long)n);
         return KSH_SUCCESS;
