@@ -377,9 +377,10 @@ __global__ void k_vector_gather(const InstDesc *descs, int ninst, int total)
 // Deterministic gather of element matrices into the lower band: one thread per band slot sums
 // the (p+1)^2 candidate elements that could hold both control points, in a fixed order (ey, then
 // ex, ascending); candidates outside the patch or not sharing both points are masked loads (0 from
-// beyond the buffer resource), so every thread issues the same straight-line loads.  About 5 in 6
-// slots of a p = 3 band are reached by no element; when the destination is known to hold 0 there
-// (skip_zero) those threads store nothing, which leaves the same bits as storing the 0 sum.
+// beyond the buffer resource), so every thread issues the same straight-line loads.  Two thirds
+// to five sixths of the slots of a p = 3 band (16 to 48 elements a side) are reached by no
+// element; when the destination is known to hold 0 there (skip_zero) those threads store
+// nothing, which leaves the same bits as storing the 0 sum.
 template <int P, typename T>
 __global__ __launch_bounds__(512) void k_band_gather(const InstDesc *descs, int ninst, long long total)
 {
