@@ -3,7 +3,7 @@
 // The reference's hook for a predictor is the protected virtual gsALMCrisfield::predictor()
 // (src/gsALMSolvers/gsALMCrisfield.h:85,100; gsALMCrisfield.hpp:111-164); this subclass
 // overrides it and leaves everything else -- the corrector iterations, the arc-length roots, the
-// residual norms that judge convergence, the stability count -- to gsALMCrisfield unchanged.
+// residual norms that decide convergence, the stability count -- to gsALMCrisfield unchanged.
 //
 // Proposal: the next state extrapolated from the last three converged states,
 // (U, lambda)_ext = 3 (U, lambda)_t - 3 (U, lambda)_{t-1} + (U, lambda)_{t-2}, corrected by the
