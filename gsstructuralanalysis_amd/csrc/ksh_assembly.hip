@@ -572,9 +572,13 @@ int run_tangent(hipStream_t s, InstDesc *d_desc, std::vector<InstDesc> &h, bool 
     KSH_HIP(hipGetLastError());
     ProfScope ps(mass ? "mass_gather" : "band_gather", s);
     {
-        const double es = f64 ? 8.0 : 4.0;  // element tiles read, band slots written
+        // the lower triangle of each element matrix read (all the gather reads), band slots written
+        const double es = f64 ? 8.0 : 4.0;
         double work = 0.0;
-        for (const auto &d : h) work += (double)d.nel * kKeTile * es + (double)d.nfree * (d.bw + 1.0) * es;
+        for (const auto &d : h) {
+            const double ne = 3.0 * (d.p + 1) * (d.p + 1);
+            work += (double)d.nel * ne * (ne + 1.0) / 2.0 * es + (double)d.nfree * (d.bw + 1.0) * es;
+        }
         ps.set_work(work);
     }
     // variants: 128, 256 or 512 threads per block (one band slot per thread either way)
