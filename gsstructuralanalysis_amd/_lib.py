@@ -41,7 +41,8 @@ class AlmOptions(C.Structure):
                 ("relaxation", C.c_double), ("quasi", C.c_int), ("quasi_iterations", C.c_int),
                 ("adaptive", C.c_int), ("adaptive_iterations", C.c_int), ("max_refine", C.c_int),
                 ("refine_tol", C.c_double), ("max_halvings", C.c_int), ("verbose", C.c_int),
-                ("inertia_guard", C.c_double), ("learned", C.c_void_p), ("refine_accept", C.c_double)]
+                ("inertia_guard", C.c_double), ("learned", C.c_void_p), ("refine_accept", C.c_double),
+                ("learned_mode", C.c_int)]
 
 
 class AlmResult(C.Structure):

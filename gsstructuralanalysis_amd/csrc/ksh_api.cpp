@@ -148,7 +148,7 @@ void destroy_problem(Problem *pb)
     if (!pb) return;
     void *ptrs[] = {pb->ctrl0, pb->free_idx, pb->free2full, pb->tabx, pb->taby, pb->wq, pb->gpgeo,
                     pb->u_full, pb->u_free, pb->r_free, pb->Ke, pb->Hg, pb->Fe, pb->F, pb->pl_N, pb->pl_span, pb->pl_F,
-                    pb->d_desc};
+                    pb->d_desc, pb->modal_P, pb->modal_Phi, pb->modal_zfree};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (pb->stream) (void)hipStreamDestroy(pb->stream);
@@ -205,6 +205,41 @@ int inertia_fp64(Matrix &K, int64_t *neg, double *dmin)
 }  // namespace
 
 int refactor_fp64(Matrix &K) { return inertia_fp64(K, nullptr, nullptr); }
+
+// Three power steps z <- z - M^-1 K z = M^-1 E z with the FP32 factor M = K + E of K, from a fixed
+// pseudo-random start; the contraction of the last step estimates rho(M^-1 E), the quantity the
+// pivot-count certificate compares with the guard.
+int power_contraction(Matrix &K)
+{
+    hipStream_t s = K.pb->stream;
+    const int n = K.n;
+    double *z = K.work + 2 * (size_t)n, *y = K.work + 3 * (size_t)n;
+    std::vector<double> h(n);
+    uint64_t st = 0x9E3779B97F4A7C15ull;
+    for (int i = 0; i < n; ++i) {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        h[i] = (double)(st >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    }
+    KSH_HIP(hipMemcpyAsync(z, h.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    int rc;
+    double prev = 0.0;
+    if ((rc = vec_norm2(s, n, z, K.red, &prev))) return rc;
+    for (int k = 0; k < 3; ++k) {
+        if ((rc = band_symv(K, z, y)) || (rc = band_solve(K, K.fac, KSH_FP32, y)) || (rc = vec_sub(s, n, z, y, z))) return rc;
+        double nz = 0.0;
+        if ((rc = vec_norm2(s, n, z, K.red, &nz))) return rc;
+        const double ratio = prev > 0 ? nz / prev : 0.0;
+        if (!std::isfinite(ratio)) {
+            K.rho = INFINITY;
+            break;
+        }
+        K.rho = ratio;  // the last step's contraction
+        prev = nz;
+        if (nz == 0) break;
+    }
+    K.rho_obs = 3;
+    return KSH_SUCCESS;
+}
 
 int ensure_fac64(Matrix &K)
 {
@@ -613,20 +648,18 @@ int ksh_factor(ksh_matrix *K)
     if (!K) return fail(KSH_OTHER_ERROR, "null argument");
     if (K->factored && K->factored_version == K->version) return K->info ? KSH_SOLVER_ERROR : KSH_SUCCESS;
     K->use64 = false;
+    K->rho = 0.0;
+    K->rho_obs = 0;
     double st[4] = {0, 0, 0, 0};
     int rc = band_factor(*K, K->fac, K->fprec, st);
     K->factored = true;
     K->factored_version = K->version;
     for (int i = 0; i < 4; ++i) K->fstats[i] = st[i];
     K->has_stats = rc == KSH_SUCCESS;
-    if (K->fprec == KSH_FP32) {
-        // pivots too small relative to the largest one cannot be trusted in FP32: use an FP64
-        // factor of the same matrix for this version's solves and inertia
-        bool fallback = rc != KSH_SUCCESS || !(st[3] >= K->guard * st[2]);
-        if (fallback) {
-            rc = inertia_fp64(*K, nullptr, nullptr);
-            K->info = (rc == KSH_SUCCESS) ? 0 : 1;
-        }
+    if (K->fprec == KSH_FP32 && rc != KSH_SUCCESS) {
+        // a zero or non-finite FP32 pivot: an FP64 factor of the same matrix serves this version
+        rc = inertia_fp64(*K, nullptr, nullptr);
+        K->info = (rc == KSH_SUCCESS) ? 0 : 1;
     }
     return rc;
 }
@@ -645,7 +678,7 @@ int ksh_solve_dev(ksh_matrix *K, const double *b_dev, double *x_dev, int max_ref
     if (max_refine > 0) {
         double *bcopy = K->work + 2 * n, *r = K->work + 3 * n;
         if (x_dev == b_dev) return fail(KSH_OTHER_ERROR, "refinement needs distinct b and x");
-        double bnorm = 0.0;
+        double bnorm = 0.0, rprev = 0.0;
         if ((rc = vec_norm2(s, n, b_dev, K->red, &bnorm))) return rc;
         KSH_HIP(hipMemcpyAsync(bcopy, b_dev, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         for (int it = 0; it < max_refine; ++it) {
@@ -655,6 +688,12 @@ int ksh_solve_dev(ksh_matrix *K, const double *b_dev, double *x_dev, int max_ref
             double rn = 0.0;
             if ((rc = vec_norm2(s, n, r, K->red, &rn))) return rc;
             if (!(rn <= tol * bnorm) && !std::isfinite(rn)) return fail(KSH_SOLVER_ERROR, "refinement diverged");
+            if (fp == KSH_FP32 && bnorm > 0 && std::isfinite(rn)) {
+                // residual contraction of this sweep: an estimate of rho(M^-1 E) for the count
+                K->rho = std::max(K->rho, it == 0 ? rn / bnorm : (rprev > 0 ? rn / rprev : 0.0));
+                ++K->rho_obs;
+            }
+            rprev = rn;
             if (rn <= tol * bnorm) break;
             if (it == max_refine - 1 && fp == KSH_FP32 && rn > K->accept * bnorm) {
                 // refinement stalled on the FP32 factor: solve with an FP64 factor instead
@@ -696,6 +735,10 @@ int ksh_inertia(ksh_matrix *K, int64_t *n_negative, double *d_min)
     if (!K) return fail(KSH_OTHER_ERROR, "null argument");
     int rc;
     if ((rc = ksh_factor(K))) return rc;
+    if (K->fprec == KSH_FP32 && !K->use64 && K->guard > 0) {
+        if (K->rho_obs == 0 && (rc = power_contraction(*K))) return rc;
+        if (K->rho > K->guard && (rc = inertia_fp64(*K, nullptr, nullptr))) return rc;
+    }
     int64_t neg = 0;
     double dmin = 0, dmaxabs = 0, dminabs = 0;
     if (K->has_stats) {  // statistics reduced by the factorization kernel itself

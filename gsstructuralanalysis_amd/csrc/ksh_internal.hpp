@@ -79,10 +79,13 @@ struct Problem {
     double fsurf[3] = {0, 0, 0};
     int device = 0;
     hipStream_t stream = nullptr;
-    double inertia_guard = 1e-6;   // guard given to the matrices created for this problem
+    double inertia_guard = 0.5;    // pivot-count certificate threshold given to new matrices
     double refine_accept = 1e-6;   // FP32-factor solves above this relative residual go FP64
     std::vector<int> h_free2full;  // host copy of free2full
     std::shared_ptr<void> modal;   // learned-predictor modal basis (built on first use)
+    double *modal_P = nullptr;     // device copies of the modal basis: [64][ncp] projector,
+    double *modal_Phi = nullptr;   // [ncp][64] mode values, [ncp] z-dof map
+    int *modal_zfree = nullptr;
 
     double *ctrl0 = nullptr;
     int *free_idx = nullptr;
@@ -122,8 +125,10 @@ struct Matrix {
     uint64_t version = 0;          // bumped on every change of ab
     uint64_t factored_version = ~0ull;
     int info = 0;                  // 0 or 1-based index of a zero / non-finite pivot
-    double guard = 1e-6;
+    double guard = 0.5;            // FP32 pivot count trusted while rho <= guard (guard <= 0: always)
     double accept = 1e-6;          // refined FP32 solves above accept ||b|| are redone in FP64
+    double rho = 0.0;              // largest residual contraction seen in refinement sweeps of this
+    int rho_obs = 0;               // FP32 factor (estimate of rho(M^-1 E), M = K + E) and their count
     double fstats[4] = {0, 0, 0, 0};  // pivot statistics of the current factor (negatives, min d, max|d|, min|d|)
     bool has_stats = false;
     // device workspace for solves and inertia

@@ -473,3 +473,16 @@ int ksh_mlp_train(ksh_mlp *m, const float *x, const float *t, int nsamples, int 
 }
 
 }  // extern "C"
+
+namespace ksh {
+
+// input and output widths of a network (the path engine checks them against its feature layout)
+int mlp_io_widths(const ksh_mlp *m, int *in, int *out)
+{
+    if (!m) return fail(KSH_OTHER_ERROR, "null network");
+    *in = m->width[0];
+    *out = m->width[Mlp::kLayers];
+    return KSH_SUCCESS;
+}
+
+}  // namespace ksh

@@ -154,3 +154,23 @@ extern "C" int ksh_predictor_decode(ksh_problem *pb, const double *coef, double 
     ksh::modal_to_free(*pb, coef, scale, dU);
     return KSH_SUCCESS;
 }
+
+namespace ksh {
+
+// device copies of the modal basis (uploaded on first use; freed with the problem)
+int modal_device(Problem &pb)
+{
+    if (pb.modal_P) return KSH_SUCCESS;
+    const ModalBasis &mb = modal_basis(pb);
+    const size_t nc = (size_t)mb.ncp * kCoef;
+    KSH_HIP(hipMalloc(&pb.modal_P, nc * sizeof(double)));
+    KSH_HIP(hipMalloc(&pb.modal_Phi, nc * sizeof(double)));
+    KSH_HIP(hipMalloc(&pb.modal_zfree, (size_t)mb.ncp * sizeof(int)));
+    KSH_HIP(hipMemcpyAsync(pb.modal_P, mb.P.data(), nc * sizeof(double), hipMemcpyHostToDevice, pb.stream));
+    KSH_HIP(hipMemcpyAsync(pb.modal_Phi, mb.Phi.data(), nc * sizeof(double), hipMemcpyHostToDevice, pb.stream));
+    KSH_HIP(hipMemcpyAsync(pb.modal_zfree, mb.zfree.data(), (size_t)mb.ncp * sizeof(int), hipMemcpyHostToDevice, pb.stream));
+    KSH_HIP(hipStreamSynchronize(pb.stream));
+    return KSH_SUCCESS;
+}
+
+}  // namespace ksh

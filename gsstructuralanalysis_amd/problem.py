@@ -29,13 +29,14 @@ def _f64(a):
 def alm_options(nsteps, length, method=0, precision=L.FP64, max_iter=20, tol=1e-6, tol_u=1e-6, tol_f=1e-3,
                 scaling=-1.0, angle_method=0, bifurcation_method=0, relaxation=1.0, quasi=0, quasi_iterations=-1,
                 adaptive=0, adaptive_iterations=10, max_refine=0, refine_tol=1e-14, max_halvings=10, verbose=0,
-                inertia_guard=0.0, learned=None, refine_accept=0.0):
+                inertia_guard=0.0, learned=None, refine_accept=0.0, learned_mode=0):
     """ksh_alm_options with the reference's option names in snake case (gsALMBase.hpp:22-54);
-    `learned` is a LearnedPredictor (Crisfield only)."""
+    `learned` is a LearnedPredictor (Crisfield only); learned_mode 1 keeps the predictor's
+    extrapolation and leaves out the network's correction (ablation)."""
     return L.AlmOptions(method, precision, nsteps, length, max_iter, tol, tol_u, tol_f, scaling, angle_method,
                         bifurcation_method, relaxation, quasi, quasi_iterations, adaptive, adaptive_iterations,
                         max_refine, refine_tol, max_halvings, verbose, inertia_guard,
-                        learned.handle if learned is not None else None, refine_accept)
+                        learned.handle if learned is not None else None, refine_accept, learned_mode)
 
 
 def _alm_result(nsteps, nfree, keep_U, probe_dof, features=False):
@@ -72,8 +73,8 @@ def _alm_collect(r, out, U, nsteps, nfree):
 
 
 def run_alm_batch(problems, nsteps, length, keep_U=False, features=False, **kw):
-    """Arc-length paths of many problems at once (ksh_batch_run_alm): each runs the unchanged
-    solver loop in its own host thread; device calls are merged into batched launches.
+    """Arc-length paths of many problems at once (ksh_batch_run_alm): one host thread advances
+    every path by one predictor or corrector iteration per round, in batched launches.
     Returns (list of per-problem result dicts, wall seconds of the whole batch)."""
     o = alm_options(nsteps, length, **kw)
     n = len(problems)

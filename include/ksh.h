@@ -44,8 +44,9 @@ enum ksh_residual_conv {
 enum ksh_precision {
     KSH_FP64 = 0,     /* FP64 assembly, FP64 band LDL^T                                   */
     KSH_FP32_IR = 1,  /* FP64 assembly, FP32 band LDL^T on MFMA, refinement residuals in
-                         FP64 against the FP64 K; near-singular FP32 pivots (|d|min <
-                         guard |d|max) switch that factor to FP64                         */
+                         FP64 against the FP64 K; the FP32 pivot count is used only where
+                         the refinement certifies it (see ksh_inertia), else an FP64 factor
+                         of the same matrix counts                                        */
     KSH_FP32 = 2      /* FP32 element GEMMs and FP32 band (K itself rounded to FP32),
                          FP32 LDL^T, refinement residuals in FP64 arithmetic on K32       */
 };
@@ -132,10 +133,14 @@ int ksh_factor(ksh_matrix *K);
 int ksh_solve(ksh_matrix *K, const double *b, double *x, int max_refine, double tol);
 int ksh_solve_dev(ksh_matrix *K, const double *b_dev, double *x_dev, int max_refine, double tol);
 /* countNegatives(vectorD()) and min(D) (src/gsALMSolvers/gsALMHelper.h:30-42,
- * src/gsALMSolvers/gsALMBase.hpp:612-614); in FP32_IR mode a near-zero pivot
- * (|d| < guard * max|d|) triggers an FP64 refactorization before counting. */
+ * src/gsALMSolvers/gsALMBase.hpp:612-614).  FP32 factors M = K + E: the count of M equals
+ * the count of K whenever rho(M^-1 E) < 1 (K(t) = M (I - t M^-1 E) stays nonsingular on
+ * t in [0, 1]).  rho is estimated from the residual contractions of refinement sweeps with
+ * M (the solves of this factor, else three power steps here); when the estimate exceeds the
+ * matrix's guard (default 0.5) the count comes from an FP64 factor of K. */
 int ksh_inertia(ksh_matrix *K, int64_t *n_negative, double *d_min);
 int ksh_vector_d(ksh_matrix *K, double *d_out);
+/* guard > 0: certificate threshold on the rho estimate; guard <= 0: FP32 counts used as they are */
 int ksh_set_inertia_guard(ksh_matrix *K, double guard);
 
 /* ---- measurement --------------------------------------------------------------
@@ -159,9 +164,11 @@ int ksh_profile_query_work(const char *kernel, double *work);
 int ksh_tune_choice(const char *kernel, int64_t *variant, int64_t *variants);
 
 /* ---- solver drivers ---------------------------------------------------------
- * The reference's solver classes (gsALMCrisfield / gsALMRiks, gsStaticNewton +
- * gsControlDisplacement) run on the std::function operator surface bound to this
- * problem, inside the same driver loops the reference's programs use. */
+ * The path engine: the algorithms of gsALMCrisfield / gsALMRiks (arc length) and of
+ * gsStaticNewton under gsControlDisplacement (load control) over this library's device
+ * operators, inside the driver loop of the reference's programs (halve the arc length and
+ * retry from the last converged point on NotConverged / AssemblyError).  Every vector stays
+ * in device memory; one host thread drives any number of problems in batched launches. */
 typedef struct {
     int method;              /* 0 Crisfield (gsALMCrisfield), 1 Riks (gsALMRiks)         */
     int precision;           /* KSH_FP64 | KSH_FP32_IR                                   */
@@ -179,15 +186,17 @@ typedef struct {
     double refine_tol;       /* stop refinement at ||b - K x|| <= refine_tol ||b||         */
     int max_halvings;        /* driver: halve the arc length on failure, retry            */
     int verbose;
-    double inertia_guard;    /* FP32 factors: a factor whose min |d| < guard max |d| is   */
-                             /* replaced by an FP64 one for its solves and pivot count    */
-                             /* (0: the library default, 1e-6)                            */
+    double inertia_guard;    /* FP32 factors: pivot-count certificate threshold on the    */
+                             /* estimated rho(M^-1 E) (ksh_inertia); 0: the library       */
+                             /* default 0.5; < 0: FP32 counts used as they are            */
     struct ksh_mlp *learned; /* Crisfield only: learned predictor network (NULL: tangent   */
                              /* predictor); a step that fails from the learned proposal   */
                              /* is redone from the tangent predictor                      */
     double refine_accept;    /* FP32 factors: a solve whose residual is still above        */
                              /* refine_accept ||b|| after max_refine sweeps is redone      */
                              /* with an FP64 factor (0: the library default, 1e-6)         */
+    int learned_mode;        /* with `learned`: 0 network-corrected extrapolation, 1 the    */
+                             /* quadratic extrapolation alone (network output taken as 0)  */
 } ksh_alm_options;
 
 typedef struct {
@@ -215,10 +224,11 @@ typedef struct {
 /* benchmarks/benchmark_Roof.cpp:320-512: options, initialize, step loop with halving. */
 int ksh_run_alm(ksh_problem *pb, const ksh_alm_options *opt, ksh_alm_result *res);
 
-/* Many independent problems (uneven sizes, same degree) each run ksh_run_alm's loop with the
- * unchanged solver classes; their device calls are merged into batched launches (assembly over
- * the union of all elements, one LDL^T / solve workgroup per band).  res[count]; wall_seconds
- * is the whole batch's wall time.  No counterpart in the reference, which runs one problem per
+/* Many independent problems (uneven sizes, same degree) each run ksh_run_alm's loop; every
+ * round advances each live path by one predictor or corrector iteration in batched launches
+ * (assembly over the union of all elements, one LDL^T / solve workgroup per band).  Each
+ * path's numerics are those of its single-problem run.  res[count]; wall_seconds is the whole
+ * batch's wall time.  No counterpart in the reference, which runs one problem per
  * process (the closest is APALM's path-interval parallelism, src/gsALMSolvers/gsAPALM.hpp:251-430). */
 int ksh_batch_run_alm(ksh_problem *const *pbs, int count, const ksh_alm_options *opt, ksh_alm_result *res,
                       double *wall_seconds);
@@ -249,8 +259,9 @@ typedef struct {
     int64_t jacobians;
 } ksh_static_result;
 
-/* examples/example_DisplacementControl.cpp:236-304 with a load-controlled residual:
- * K_L, F from assemble(); gsStaticNewton(K_L, F, Jacobian, ALResidual); gsControlDisplacement. */
+/* examples/example_DisplacementControl.cpp:236-304 with a load-controlled residual: Newton
+ * (src/gsStaticSolvers/gsStaticNewton.hpp:142-193) for each load step of
+ * gsControlDisplacement::step (src/gsStaticSolvers/gsControlDisplacement.h:46-60). */
 int ksh_run_static(ksh_problem *pb, const ksh_static_options *opt, ksh_static_result *res);
 
 /* ---- eigen-solvers --------------------------------------------------------------

@@ -271,6 +271,11 @@ class ALMCrisfield:
     def set_solution(self, U, L):
         self.U, self.L = U.copy(), L
 
+    def set_length(self, length):
+        """setLength (gsALMBase.h:118-122): the arc length and its 'previous' value."""
+        self.arc = length
+        self.arc_prev = length
+
 
 class ALMRiks(ALMCrisfield):
     """gsALMRiks (gsALMRiks.hpp:22-134,205-212): linearised constraint with phi = 1/n."""
@@ -342,7 +347,7 @@ def run_crisfield(patch, nsteps, length, max_halvings=10, method=0, **kw):
                 break
             halvings += 1
             dL /= 2.0
-            alm.arc = dL
+            alm.set_length(dL)
             alm.set_solution(Uold, Lold)
             bisected = True
             continue
@@ -359,7 +364,7 @@ def run_crisfield(patch, nsteps, length, max_halvings=10, method=0, **kw):
         out["residue_f"].append(alm.residueF)
         if not bisected:
             dL = dL0
-            alm.arc = dL
+            alm.set_length(dL)
         bisected = False
         k += 1
     res = {key: np.array(v) for key, v in out.items()}
