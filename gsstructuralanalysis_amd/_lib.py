@@ -42,7 +42,7 @@ class AlmOptions(C.Structure):
                 ("adaptive", C.c_int), ("adaptive_iterations", C.c_int), ("max_refine", C.c_int),
                 ("refine_tol", C.c_double), ("max_halvings", C.c_int), ("verbose", C.c_int),
                 ("inertia_guard", C.c_double), ("learned", C.c_void_p), ("refine_accept", C.c_double),
-                ("learned_mode", C.c_int)]
+                ("learned_mode", C.c_int), ("lengths", _dp)]
 
 
 class AlmResult(C.Structure):

@@ -153,7 +153,7 @@ private:
         if (o.method != 0 && o.method != 1) return fail(KSH_OTHER_ERROR, "unknown arc-length method");
         if (o.precision != KSH_FP64 && o.precision != KSH_FP32_IR && o.precision != KSH_FP32)
             return fail(KSH_OTHER_ERROR, "unknown precision");
-        if (o.nsteps < 0 || o.max_iter < 2 || !(o.length > 0)) return fail(KSH_OTHER_ERROR, "bad step options");
+        if (o.nsteps < 0 || o.max_iter < 2 || (!o.lengths && !(o.length > 0))) return fail(KSH_OTHER_ERROR, "bad step options");
         if (o.quasi || o.quasi_iterations > 0)
             return fail(KSH_OTHER_ERROR, "quasi-Newton iterations are not supported by the path engine");
         if (o.bifurcation_method != -1 && o.bifurcation_method != 0)
@@ -195,7 +195,8 @@ private:
             if ((rc = ksh_matrix_create(static_cast<ksh_problem *>(I.pb), m_o.precision, &K))) return rc;
             m_mats.ms.push_back(K);
             I.K = K;
-            I.arc = I.arc0 = I.arc_prev = m_o.length;
+            I.arc = I.arc0 = I.arc_prev = m_o.lengths ? m_o.lengths[i] : m_o.length;
+            if (!(I.arc > 0)) return fail(KSH_OTHER_ERROR, "arc lengths must be positive");
             I.phi = m_o.scaling;
             I.phi_user = m_o.scaling != -1;
             if (m_o.learned && (rc = modal_device(*I.pb))) return rc;
@@ -861,7 +862,7 @@ private:
                 std::printf("[ksh path] step %d: lambda %.10e |U| %.6e iterations %d negatives %lld length %.3e\n", k,
                             I->L, I->unorm_new, I->it, (long long)I->negatives, arc_used);
             // the driver: arc length back to its start value after a step that was not halved
-            if (!I->bisected) I->arc = I->arc_prev = I->arc0 = m_o.length;
+            if (!I->bisected) I->arc = I->arc_prev = I->arc0;
             I->bisected = false;
             I->learned_off = false;
             I->k = k + 1;
@@ -1153,7 +1154,12 @@ extern "C" int ksh_run_alm(ksh_problem *pb, const ksh_alm_options *o, ksh_alm_re
 {
     if (!pb || !o || !res) return ksh::fail(KSH_OTHER_ERROR, "null argument");
     return ksh::guarded([&] {
-        ksh::AlmEngine e(*o);
+        ksh_alm_options one = *o;
+        if (one.lengths) {
+            one.length = one.lengths[0];
+            one.lengths = nullptr;
+        }
+        ksh::AlmEngine e(one);
         ksh_problem *arr[1] = {pb};
         double wall = 0;
         return e.run(arr, 1, res, &wall);

@@ -245,6 +245,20 @@ def gaussian_random_field(points: np.ndarray, corr_len: float, rng: np.random.Ge
     return Lc @ rng.standard_normal(len(points))
 
 
+# Arc length of the shallow-roof benchmark.  The reference runs the roof in millimetres with
+# Scaling = 0 (a displacement-only arc length) and dL = 0.5 on its default mesh: degree 2 elevated
+# once and refined once, 2 x 2 elements of degree 3, 45 free dofs (benchmarks/benchmark_Roof.cpp:
+# 44-76, 365).  Here lengths are in metres, and ||dU|| over n free dofs of a smooth field grows
+# like sqrt(n), so the same physical step is 0.5e-3 * sqrt(n / 45).  `kappa` scales it (1: the
+# reference's step).
+REF_ARC_LENGTH_M = 0.5e-3
+REF_FREE_DOFS = 45
+
+
+def roof_arc_length(nfree: int, kappa: float = 1.0) -> float:
+    return kappa * REF_ARC_LENGTH_M * float(np.sqrt(nfree / REF_FREE_DOFS))
+
+
 def imperfect_roof_instance(i: int, sizes=(16, 24, 32, 40, 48), p: int = 3, seed0: int = 20261202,
                             corr_len: float = 0.1, amp_frac: float = 0.5, **kw) -> Patch:
     """Instance i of the batched imperfection sweep on the shallow-roof panel family.

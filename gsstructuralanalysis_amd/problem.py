@@ -76,7 +76,15 @@ def run_alm_batch(problems, nsteps, length, keep_U=False, features=False, **kw):
     """Arc-length paths of many problems at once (ksh_batch_run_alm): one host thread advances
     every path by one predictor or corrector iteration per round, in batched launches.
     Returns (list of per-problem result dicts, wall seconds of the whole batch)."""
+    lengths = None
+    if np.ndim(length) > 0:  # one arc length per problem
+        lengths = np.ascontiguousarray(length, dtype=np.float64)
+        if lengths.shape != (len(problems),):
+            raise ValueError("one arc length per problem")
+        length = float(lengths[0])
     o = alm_options(nsteps, length, **kw)
+    if lengths is not None:
+        o.lengths = _dptr(lengths)
     n = len(problems)
     handles = (C.c_void_p * n)(*[p.handle for p in problems])
     rs = (L.AlmResult * n)()

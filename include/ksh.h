@@ -197,6 +197,8 @@ typedef struct {
                              /* with an FP64 factor (0: the library default, 1e-6)         */
     int learned_mode;        /* with `learned`: 0 network-corrected extrapolation, 1 the    */
                              /* quadratic extrapolation alone (network output taken as 0)  */
+    const double *lengths;   /* ksh_batch_run_alm: arc length of each problem (NULL: all    */
+                             /* take `length`)                                              */
 } ksh_alm_options;
 
 typedef struct {

@@ -327,9 +327,12 @@ class ALMRiks(ALMCrisfield):
         self.arc_prev = self.arc
 
 
-def run_crisfield(patch, nsteps, length, max_halvings=10, method=0, **kw):
-    """Driver loop of benchmarks/benchmark_Roof.cpp:417-512 on the oracle (method 0 Crisfield, 1 Riks)."""
-    ops = OracleOps(patch)
+def run_crisfield(patch, nsteps, length, max_halvings=10, method=0, ops=None, keep_jac=False, **kw):
+    """Driver loop of benchmarks/benchmark_Roof.cpp:417-512 on the oracle (method 0 Crisfield, 1 Riks).
+
+    ``ops``: any object with the OracleOps interface (default: the shell oracle of ``patch``);
+    ``keep_jac``: also return the counted tangent (band) of every converged step."""
+    ops = ops if ops is not None else OracleOps(patch)
     alm = (ALMCrisfield if method == 0 else ALMRiks)(ops, length, **kw)
     t0 = time.perf_counter()
     alm.initialize()
@@ -337,7 +340,9 @@ def run_crisfield(patch, nsteps, length, max_halvings=10, method=0, **kw):
     dL0 = dL = length
     bisected = False
     out = {"lambda": [], "unorm": [], "negatives": [], "indicator": [], "iterations": [], "length": [], "U": [],
-           "residue_f": []}
+           "residue_f": [], "phi_step": []}
+    if keep_jac:
+        out["jac"] = []
     halvings = 0
     k = 0
     while k < nsteps:
@@ -362,6 +367,9 @@ def run_crisfield(patch, nsteps, length, max_halvings=10, method=0, **kw):
         out["length"].append(dL)
         out["U"].append(Uold)
         out["residue_f"].append(alm.residueF)
+        out["phi_step"].append(alm.phi)
+        if keep_jac:
+            out["jac"].append(np.array(alm.jac, copy=True))
         if not bisected:
             dL = dL0
             alm.set_length(dL)
