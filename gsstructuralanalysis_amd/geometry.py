@@ -248,9 +248,11 @@ def gaussian_random_field(points: np.ndarray, corr_len: float, rng: np.random.Ge
 # Arc length of the shallow-roof benchmark.  The reference runs the roof in millimetres with
 # Scaling = 0 (a displacement-only arc length) and dL = 0.5 on its default mesh: degree 2 elevated
 # once and refined once, 2 x 2 elements of degree 3, 45 free dofs (benchmarks/benchmark_Roof.cpp:
-# 44-76, 365).  Here lengths are in metres, and ||dU|| over n free dofs of a smooth field grows
-# like sqrt(n), so the same physical step is 0.5e-3 * sqrt(n / 45).  `kappa` scales it (1: the
-# reference's step).
+# 44-76, 365).  Here lengths are in metres, and the displacement norm over n free dofs of the same
+# physical motion grows like sqrt(n): at one fixed arc length on the imperfect-roof batch the
+# median step of the first negative pivot is 15, 24, 30, 36, 45 for 16, 24, 32, 40, 48 elements a
+# side, in proportion to sqrt(n_free).  So the same physical step is 0.5e-3 * sqrt(n / 45);
+# `kappa` scales it.
 REF_ARC_LENGTH_M = 0.5e-3
 REF_FREE_DOFS = 45
 

@@ -27,7 +27,7 @@ def test_roofline_takes_the_largest_main_stream_kernel(bench):
     # largest total but overlap the main stream, so the FP32 factor is the dominant kernel
     stats = {"band_ldlt_batch_fp32": (222, 1875.0, 222 * 92.07e9), "band_ldlt_early_fp64": (155, 1900.0, 155 * 10e9),
              "band_ldlt_batch_fp64": (212, 910.0, 212 * 12e9), "band_gather": (222, 780.0, 222 * 4e9)}
-    r = bench.roofline(stats, True)
+    r = bench.roofline(stats)
     assert r["kernel"] == "band_ldlt_batch_fp32" and r["bound"] == "mfma" and r["unit"] == "TFLOP/s"
     assert r["peak"] == 157.3 and abs(r["achieved"] - 92.07e9 / (1875.0 / 222 / 1e3) / 1e12) < 1e-9
     assert abs(r["frac"] - r["achieved"] / r["peak"]) < 1e-12
@@ -36,6 +36,6 @@ def test_roofline_takes_the_largest_main_stream_kernel(bench):
 
 def test_roofline_of_an_hbm_kernel(bench):
     stats = {"band_gather": (10, 35.0, 10 * 4e9), "band_solve_batch_fp32": (5, 1.0, 5 * 1e9)}
-    r = bench.roofline(stats, False)
+    r = bench.roofline(stats)
     assert r["kernel"] == "band_gather" and r["bound"] == "hbm" and r["unit"] == "GB/s" and r["peak"] == 8000.0
     assert abs(r["achieved"] - 4e9 / 3.5e-3 / 1e9) < 1e-9
