@@ -3,11 +3,12 @@
 // Precision handling of the mixed form lives here:
 //   - FP32 factors of the assembled FP64 (or FP32) band; a zero or non-finite FP32 pivot gets an
 //     FP64 factor of the same values;
-//   - refined solves against the assembled K with FP64 residuals; a solve still above
-//     accept ||b|| after its sweeps is redone with an FP64 factor (one batched refactorization
-//     per round, or the early FP64 factor started beside the FP32 one);
-//   - each FP32 factor M = K + E records the largest residual contraction its refinement sweeps
-//     showed, an estimate of rho(M^-1 E).  While rho(M^-1 E) < 1 the path K(t) = M - t E,
+//   - solves with an FP32 factor run FGMRES in FP64 against the assembled K, preconditioned by the
+//     factor (GMRES-IR); a solve still above its tolerance after its Krylov steps is redone with an
+//     FP64 factor (one batched refactorization per round, or the early FP64 factor started
+//     beside the FP32 one);
+//   - each FP32 factor M = K + E records the largest residual contraction its solves showed, an
+//     estimate of rho(M^-1 E).  While rho(M^-1 E) < 1 the path K(t) = M - t E,
 //     t in [0, 1], stays nonsingular (K(t) = M (I - t M^-1 E)), so no eigenvalue crosses zero and
 //     M has the inertia of K: the FP32 pivot count is exact.  The engine trusts the count when the
 //     estimate is below the matrix's guard (default 0.5, a factor 2 of margin on the estimate) and
@@ -339,17 +340,30 @@ int BatchDevice::residual(const std::vector<ResidualJob> &jobs)
     return batch_residual(m_stream, (InstDesc *)m_desc.p, h);
 }
 
+// Solves K x = b.  FP64 factors: direct solve (plus classical refinement sweeps if asked).  FP32
+// factors M = K + E: FGMRES in FP64 on K with M^-1 as the preconditioner (GMRES-IR): x0 = M^-1 b,
+// r0 = b - K x0, then at most max_refine Arnoldi steps z_j = M^-1 v_j, u = K z_j, orthogonalised
+// against v_0..v_j by classical Gram-Schmidt with one reorthogonalisation, Givens rotations for
+// the least-squares residual |g_j+1| = ||b - K x_j||; x = x0 + Z y.  Refinement sweeps stall once
+// rho(M^-1 E) approaches 1 (thin shells near limit points); the Krylov space takes the few
+// outlying eigenvalues of K M^-1 out in as many steps.  A solve still above `tol` after its steps
+// is redone with an FP64 factor.  The first two residual contractions (||r0|| / ||b|| and
+// ||(I - K M^-1) v_0||, one power step of E M^-1 each) go into the factor's rho estimate.
 int BatchDevice::solve(std::vector<SolveJob> &jobs)
 {
     if (jobs.empty()) return KSH_SUCCESS;
     const size_t nj = jobs.size();
     int rc;
     if ((rc = ensure_desc(nj))) return rc;
+    int nmax = 0, mmax = 0;
     for (SolveJob &q : jobs) {
         Matrix &K = *q.K;
         if (!K.factored || K.factored_version != K.version || K.info) return fail(KSH_SOLVER_ERROR, "matrix is not factored");
         if (q.b == q.x) return fail(KSH_OTHER_ERROR, "solve needs distinct b and x");
+        if (q.max_refine > kMaxKrylov) return fail(KSH_OTHER_ERROR, "at most 8 refinement steps per solve");
         q.rc = KSH_SUCCESS;
+        nmax = std::max(nmax, K.n);
+        mmax = std::max(mmax, q.max_refine);
     }
     std::vector<size_t> off(nj), poff(nj);
     size_t tot = 0, ptot = 0;
@@ -359,8 +373,19 @@ int BatchDevice::solve(std::vector<SolveJob> &jobs)
         poff[i] = ptot;
         ptot += 2 * (((size_t)jobs[i].K->n + 63) / 64);
     }
-    if ((rc = m_r.ensure(tot * sizeof(double)))) return rc;
+    // per job: r, then the Krylov basis V (mmax + 1 vectors) and Z (mmax vectors)
+    const size_t per = 1 + (mmax > 0 ? 2 * (size_t)mmax + 1 : 0);
+    if ((rc = m_r.ensure(tot * per * sizeof(double)))) return rc;
+    if (mmax > 0 && (rc = m_part.ensure(ptot * sizeof(double)))) return rc;
+    if (mmax > 0 && (size_t)nmax > m_zero_n) {
+        if ((rc = m_zero.ensure((size_t)nmax * sizeof(double)))) return rc;
+        KSH_HIP(hipMemsetAsync(m_zero.p, 0, (size_t)nmax * sizeof(double), m_stream));
+        m_zero_n = (size_t)nmax;
+    }
     double *rbase = (double *)m_r.p;
+    auto R = [&](size_t i) { return rbase + off[i]; };
+    auto V = [&](size_t i, int k) { return rbase + tot + (size_t)off[i] * (2 * mmax + 1) + (size_t)k * jobs[i].K->n; };
+    auto Z = [&](size_t i, int k) { return V(i, mmax + 1 + k); };
     {  // x = b
         std::vector<VecDesc> cp(nj);
         for (size_t i = 0; i < nj; ++i) {
@@ -369,87 +394,278 @@ int BatchDevice::solve(std::vector<SolveJob> &jobs)
         }
         if ((rc = batch_vecops(m_stream, m_vecdesc, cp))) return rc;
     }
-    // solves of the given right-hand sides (in place), grouped by the precision of the factor in use
+    std::vector<int> efp(nj);
+    for (size_t i = 0; i < nj; ++i) efp[i] = jobs[i].K->use64 ? KSH_FP64 : jobs[i].K->fprec;
+    // in-place solves of the given right-hand sides with each job's factor in use
     auto solve_all = [&](const std::vector<double *> &rhs, const std::vector<int> &active) -> int {
         for (int fp : {KSH_FP64, KSH_FP32}) {
             std::vector<BandDescHost> h;
             for (size_t i = 0; i < nj; ++i) {
+                if (efp[i] != fp || !active[i]) continue;
                 Matrix &K = *jobs[i].K;
-                const int efp = K.use64 ? KSH_FP64 : K.fprec;
-                if (efp != fp) continue;
                 void *F = K.use64 ? (void *)K.fac64 : K.fac;
-                h.push_back(BandDescHost{F, K.n, K.bw, nullptr, nullptr, rhs[i], active[i]});
+                h.push_back(BandDescHost{F, K.n, K.bw, nullptr, nullptr, rhs[i], 1});
             }
             int rc2;
             if (!h.empty() && (rc2 = band_solve_batch(m_stream, m_bdesc.p, h, fp))) return rc2;
         }
         return KSH_SUCCESS;
     };
-    std::vector<double *> xs(nj), rs(nj);
-    for (size_t i = 0; i < nj; ++i) {
-        xs[i] = jobs[i].x;
-        rs[i] = rbase + off[i];
-    }
-    std::vector<int> active(nj, 1);
-    if ((rc = solve_all(xs, active))) return rc;
-    int max_refine = 0;
-    for (const SolveJob &q : jobs) max_refine = std::max(max_refine, q.max_refine);
-    if (max_refine > 0 && (rc = m_part.ensure(ptot * sizeof(double)))) return rc;
-    // Refinement, per job the decisions of the single-problem solve: stop at ||r|| <= tol ||b||;
-    // on the last sweep of an FP32 factor whose residual is still above accept ||b||, replace the
-    // factor by an FP64 one and solve again.  Residual ratios feed the FP32 count certificate.
-    std::vector<int> efp(nj), fallback(nj, 0);
-    std::vector<double> rprev(nj, 0.0);
-    for (size_t i = 0; i < nj; ++i) {
-        efp[i] = jobs[i].K->use64 ? KSH_FP64 : jobs[i].K->fprec;
-        if (jobs[i].max_refine <= 0) active[i] = 0;
-    }
-    for (int it = 0; it < max_refine; ++it) {
+    // r = b - K x for the active jobs (b = nullptr: the zero vector), norms to the host
+    std::vector<double> nr(2 * nj);
+    auto residuals = [&](const std::vector<const double *> &b, const std::vector<const double *> &x,
+                         const std::vector<double *> &r, const std::vector<int> &active) -> int {
         std::vector<RefDesc> h(nj);
         for (size_t i = 0; i < nj; ++i) {
             Matrix &K = *jobs[i].K;
-            h[i] = RefDesc{K.ab, K.sprec, K.n, K.bw, active[i], jobs[i].b, jobs[i].x, rs[i], (double *)m_norms.p + 2 * i,
-                           (double *)m_part.p + poff[i], 0, 0};
+            h[i] = RefDesc{K.ab, K.sprec, K.n, K.bw, active[i], b[i] ? b[i] : (const double *)m_zero.p,
+                           const_cast<double *>(x[i]), r[i], (double *)m_norms.p + 2 * i, (double *)m_part.p + poff[i], 0, 0};
         }
-        if ((rc = refine_residual_batch(m_stream, (RefDesc *)m_rdesc.p, h))) return rc;
-        std::vector<double> nr(2 * nj);
+        int rc2;
+        if ((rc2 = refine_residual_batch(m_stream, (RefDesc *)m_rdesc.p, h))) return rc2;
         KSH_HIP(hipMemcpyAsync(nr.data(), m_norms.p, sizeof(double) * nr.size(), hipMemcpyDeviceToHost, m_stream));
         KSH_HIP(hipStreamSynchronize(m_stream));
-        std::vector<int> last(nj, 0);
-        bool any = false;
-        for (size_t i = 0; i < nj; ++i) {
-            if (!active[i]) continue;
+        return KSH_SUCCESS;
+    };
+    std::vector<double *> xs(nj);
+    for (size_t i = 0; i < nj; ++i) xs[i] = jobs[i].x;
+    std::vector<int> active(nj, 1);
+    if ((rc = solve_all(xs, active))) return rc;
+    if (mmax == 0) return KSH_SUCCESS;
+
+    // ---- residual of the first solve ---------------------------------------------------------
+    std::vector<const double *> bs(nj), xc(nj);
+    std::vector<double *> rs(nj);
+    for (size_t i = 0; i < nj; ++i) {
+        bs[i] = jobs[i].b;
+        xc[i] = jobs[i].x;
+        rs[i] = R(i);
+        active[i] = jobs[i].max_refine > 0;
+    }
+    if ((rc = residuals(bs, xc, rs, active))) return rc;
+    std::vector<double> bn(nj, 0.0), beta(nj, 0.0);
+    std::vector<int> fallback(nj, 0), gm(nj, 0), ir(nj, 0);
+    for (size_t i = 0; i < nj; ++i) {
+        if (!active[i]) continue;
+        const double rn = std::sqrt(nr[2 * i]);
+        bn[i] = std::sqrt(nr[2 * i + 1]);
+        beta[i] = rn;
+        if (!std::isfinite(rn)) {
+            jobs[i].rc = fail(KSH_SOLVER_ERROR, "non-finite residual in the refined solve");
+            active[i] = 0;
+            continue;
+        }
+        if (efp[i] == KSH_FP32 && bn[i] > 0) {
             Matrix &K = *jobs[i].K;
-            const double rn = std::sqrt(nr[2 * i]), bn = std::sqrt(nr[2 * i + 1]);
-            if (efp[i] == KSH_FP32 && bn > 0 && std::isfinite(rn)) {
-                // contraction of this sweep: ||r_0|| / ||b||, then ||r_k|| / ||r_k-1||
-                const double ratio = it == 0 ? rn / bn : (rprev[i] > 0 ? rn / rprev[i] : 0.0);
-                K.rho = std::max(K.rho, ratio);
-                ++K.rho_obs;
-            }
-            rprev[i] = rn;
-            const bool conv = rn <= jobs[i].tol * bn;
-            if (!conv && !std::isfinite(rn)) {
-                jobs[i].rc = fail(KSH_SOLVER_ERROR, "refinement diverged");
-                active[i] = 0;
-            } else if (conv) {
-                active[i] = 0;
-            } else if (it == jobs[i].max_refine - 1 && efp[i] == KSH_FP32 && rn > K.accept * bn) {
-                fallback[i] = 1;
-                active[i] = 0;
-            } else {
-                last[i] = it == jobs[i].max_refine - 1;
+            K.rho = std::max(K.rho, rn / bn[i]);
+            ++K.rho_obs;
+        }
+        if (rn <= jobs[i].tol * bn[i]) active[i] = 0;
+        else if (efp[i] == KSH_FP32) gm[i] = 1;
+        else ir[i] = 1;
+    }
+    // ---- FP64 factors: classical refinement sweeps ---------------------------------------------
+    for (int it = 0; it < mmax; ++it) {
+        std::vector<int> act(nj, 0);
+        bool any = false;
+        for (size_t i = 0; i < nj; ++i)
+            if (ir[i] && it < jobs[i].max_refine) {
+                act[i] = 1;
                 any = true;
             }
-        }
         if (!any) break;
-        if ((rc = solve_all(rs, active))) return rc;
-        for (size_t i = 0; i < nj; ++i) h[i].active = active[i];
+        if ((rc = solve_all(rs, act))) return rc;
+        std::vector<RefDesc> h(nj);
+        for (size_t i = 0; i < nj; ++i) {
+            Matrix &K = *jobs[i].K;
+            h[i] = RefDesc{K.ab, K.sprec, K.n, K.bw, act[i], jobs[i].b, jobs[i].x, R(i), (double *)m_norms.p + 2 * i,
+                           (double *)m_part.p + poff[i], 0, 0};
+        }
         if ((rc = refine_update_batch(m_stream, (RefDesc *)m_rdesc.p, h))) return rc;
-        for (size_t i = 0; i < nj; ++i)
-            if (last[i]) active[i] = 0;
+        if ((rc = residuals(bs, xc, rs, act))) return rc;
+        for (size_t i = 0; i < nj; ++i) {
+            if (!act[i]) continue;
+            const double rn = std::sqrt(nr[2 * i]);
+            if (!std::isfinite(rn)) {
+                jobs[i].rc = fail(KSH_SOLVER_ERROR, "refinement diverged");
+                ir[i] = 0;
+            } else if (rn <= jobs[i].tol * bn[i]) {
+                ir[i] = 0;
+            }
+        }
     }
-    // FP64 fallback of stalled refinements: x = K64^-1 b, from the early FP64 factor where one was
+    // ---- FP32 factors: FGMRES with the FP32 factor as preconditioner -----------------------------
+    std::vector<std::vector<double>> H(nj), cs(nj), sn(nj), g(nj);
+    std::vector<int> steps(nj, 0);
+    {
+        std::vector<VecDesc> vd;
+        for (size_t i = 0; i < nj; ++i) {
+            if (!gm[i]) continue;
+            const int m = jobs[i].max_refine;
+            H[i].assign((size_t)(m + 1) * m, 0.0);
+            cs[i].assign(m, 0.0);
+            sn[i].assign(m, 0.0);
+            g[i].assign(m + 1, 0.0);
+            g[i][0] = beta[i];
+            VecDesc d;
+            d.n = jobs[i].K->n;
+            d.add(V(i, 0), 1.0 / beta[i], R(i));
+            vd.push_back(d);
+        }
+        if ((rc = batch_vecops(m_stream, m_vecdesc, vd))) return rc;
+    }
+    for (int j = 0; j < mmax; ++j) {
+        std::vector<int> act(nj, 0);
+        bool any = false;
+        for (size_t i = 0; i < nj; ++i)
+            if (gm[i] == 1 && j < jobs[i].max_refine) {
+                act[i] = 1;
+                any = true;
+            }
+        if (!any) break;
+        // z_j = M^-1 v_j ; u = K z_j (as -u = 0 - K z_j, in R)
+        std::vector<VecDesc> cp;
+        std::vector<double *> zs(nj, nullptr);
+        for (size_t i = 0; i < nj; ++i) {
+            if (!act[i]) continue;
+            VecDesc d;
+            d.n = jobs[i].K->n;
+            d.add(Z(i, j), 1.0, V(i, j));
+            cp.push_back(d);
+            zs[i] = Z(i, j);
+        }
+        if ((rc = batch_vecops(m_stream, m_vecdesc, cp)) || (rc = solve_all(zs, act))) return rc;
+        std::vector<const double *> zero(nj, nullptr), zc(nj);
+        for (size_t i = 0; i < nj; ++i) zc[i] = zs[i] ? zs[i] : jobs[i].x;
+        if ((rc = residuals(zero, zc, rs, act))) return rc;
+        // two Gram-Schmidt passes against v_0..v_j (R holds -u)
+        std::vector<double> h1(nj * (kMaxKrylov + 1), 0.0), uu(nj, 0.0);
+        for (int pass = 0; pass < 2; ++pass) {
+            std::vector<DotDesc> dd;
+            std::vector<size_t> who;
+            for (size_t i = 0; i < nj; ++i) {
+                if (!act[i]) continue;
+                DotDesc d;
+                d.n = jobs[i].K->n;
+                for (int k = 0; k <= j; ++k) {
+                    d.a[d.npairs] = R(i);
+                    d.b[d.npairs] = V(i, k);
+                    ++d.npairs;
+                }
+                d.a[d.npairs] = R(i);
+                d.b[d.npairs] = R(i);
+                ++d.npairs;
+                dd.push_back(d);
+                who.push_back(i);
+            }
+            std::vector<double> dv;
+            if ((rc = dots(dd, dv))) return rc;
+            std::vector<VecDesc> vd;
+            size_t o = 0;
+            for (size_t q = 0; q < who.size(); ++q) {
+                const size_t i = who[q];
+                const double sign = pass == 0 ? -1.0 : 1.0;  // pass 0 works on -u, pass 1 on the projected u
+                VecDesc d;
+                d.n = jobs[i].K->n;
+                std::vector<double> c(j + 1);
+                for (int k = 0; k <= j; ++k) c[k] = sign * dv[o + k];
+                const double ww = dv[o + j + 1];
+                o += (size_t)j + 2;
+                for (int k = 0; k <= j; ++k) h1[i * (kMaxKrylov + 1) + k] += c[k];
+                if (pass == 0) uu[i] = ww;
+                else uu[i] = ww;  // ||u - V h||^2 before the second projection
+                // R <- sign R - sum c_k v_k  (pass 0 turns -u into u on the way)
+                int k = 0;
+                d.add(R(i), sign, R(i), -(k <= j ? c[k] : 0.0), V(i, 0), k + 1 <= j ? -c[k + 1] : 0.0,
+                      k + 1 <= j ? V(i, k + 1) : nullptr);
+                for (k = 2; k <= j; k += 2)
+                    d.add(R(i), 1.0, R(i), -c[k], V(i, k), k + 1 <= j ? -c[k + 1] : 0.0, k + 1 <= j ? V(i, k + 1) : nullptr);
+                vd.push_back(d);
+                if (pass == 1) {
+                    double proj = 0.0;
+                    for (int kk = 0; kk <= j; ++kk) proj += c[kk] * c[kk];
+                    uu[i] = std::max(0.0, ww - proj);  // ||w||^2 after the second projection
+                }
+            }
+            if ((rc = batch_vecops(m_stream, m_vecdesc, vd))) return rc;
+            if (pass == 0) {
+                // first power step of E M^-1 on v_0: ||v_0 - u||^2 = 1 - 2 v_0.u + u.u
+                for (size_t i = 0; i < nj; ++i)
+                    if (act[i] && j == 0) {
+                        Matrix &K = *jobs[i].K;
+                        const double hh = h1[i * (kMaxKrylov + 1)];
+                        const double r2 = 1.0 - 2.0 * hh + uu[i];
+                        K.rho = std::max(K.rho, std::sqrt(std::max(0.0, r2)));
+                        ++K.rho_obs;
+                    }
+            }
+        }
+        std::vector<VecDesc> vd;
+        for (size_t i = 0; i < nj; ++i) {
+            if (!act[i]) continue;
+            const int m = jobs[i].max_refine;
+            double *Hc = H[i].data() + (size_t)j * (m + 1);  // column j
+            for (int k = 0; k <= j; ++k) Hc[k] = h1[i * (kMaxKrylov + 1) + k];
+            const double hn = std::sqrt(uu[i]);
+            Hc[j + 1] = hn;
+            for (int k = 0; k < j; ++k) {  // previous rotations
+                const double a = Hc[k], b = Hc[k + 1];
+                Hc[k] = cs[i][k] * a + sn[i][k] * b;
+                Hc[k + 1] = -sn[i][k] * a + cs[i][k] * b;
+            }
+            const double den = std::hypot(Hc[j], Hc[j + 1]);
+            cs[i][j] = den > 0 ? Hc[j] / den : 1.0;
+            sn[i][j] = den > 0 ? Hc[j + 1] / den : 0.0;
+            Hc[j] = den;
+            Hc[j + 1] = 0.0;
+            g[i][j + 1] = -sn[i][j] * g[i][j];
+            g[i][j] = cs[i][j] * g[i][j];
+            steps[i] = j + 1;
+            const double res = std::abs(g[i][j + 1]);
+            if (!std::isfinite(res)) {
+                gm[i] = 0;
+                fallback[i] = 1;
+                continue;
+            }
+            if (res <= jobs[i].tol * bn[i] || hn == 0.0) {
+                gm[i] = 2;  // converged: solution update below
+            } else if (j + 1 < m) {
+                VecDesc d;
+                d.n = jobs[i].K->n;
+                d.add(V(i, j + 1), 1.0 / hn, R(i));
+                vd.push_back(d);
+            }
+        }
+        if ((rc = batch_vecops(m_stream, m_vecdesc, vd))) return rc;
+        for (size_t i = 0; i < nj; ++i)
+            if (gm[i] == 2) gm[i] = 3;  // stop iterating, keep for the update
+    }
+    // x = x0 + Z y, R y = g (back substitution on the rotated Hessenberg)
+    {
+        std::vector<VecDesc> vd;
+        for (size_t i = 0; i < nj; ++i) {
+            if (!gm[i]) continue;
+            const int m = jobs[i].max_refine, k = steps[i];
+            if (gm[i] == 1 && std::abs(g[i][k]) > jobs[i].tol * bn[i]) {
+                fallback[i] = 1;  // not converged within the steps: FP64 factor below
+                continue;
+            }
+            std::vector<double> y(k, 0.0);
+            for (int r = k - 1; r >= 0; --r) {
+                double sum = g[i][r];
+                for (int c = r + 1; c < k; ++c) sum -= H[i][(size_t)c * (m + 1) + r] * y[c];
+                y[r] = sum / H[i][(size_t)r * (m + 1) + r];
+            }
+            VecDesc d;
+            d.n = jobs[i].K->n;
+            for (int c = 0; c < k; c += 2)
+                d.add(jobs[i].x, 1.0, jobs[i].x, y[c], Z(i, c), c + 1 < k ? y[c + 1] : 0.0, c + 1 < k ? Z(i, c + 1) : nullptr);
+            vd.push_back(d);
+            prof_count("gmres_steps", k);
+        }
+        if ((rc = batch_vecops(m_stream, m_vecdesc, vd))) return rc;
+    }
+    // FP64 fallback of unconverged solves: x = K64^-1 b, from the early FP64 factor where one was
     // started, else from one batched refactorization
     std::vector<Matrix *> re;
     for (size_t i = 0; i < nj; ++i)

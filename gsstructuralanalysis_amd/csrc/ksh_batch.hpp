@@ -13,6 +13,8 @@
 
 namespace ksh {
 
+constexpr int kMaxKrylov = 8;  // refinement (Krylov) steps per solve at most
+
 struct TangentJob {
     Problem *pb;
     Matrix *K;
@@ -87,6 +89,8 @@ private:
     void *m_hearly = nullptr;  // pinned: info + stats of the early factors
     size_t m_hearly_bytes = 0;
     DevScratch m_desc, m_bdesc, m_cdesc, m_rdesc, m_stats, m_info, m_norms, m_part, m_r, m_dotdesc, m_vecdesc, m_dotout;
+    DevScratch m_zero;
+    size_t m_zero_n = 0;
     long long m_refactors = 0;
 };
 

@@ -8,7 +8,7 @@
 
 namespace ksh {
 
-constexpr int kMaxDots = 8;
+constexpr int kMaxDots = 10;
 constexpr int kMaxVecOps = 6;
 constexpr int kModalCoef = 64;   // 8 x 8 analytic out-of-plane modes of the learned predictor
 constexpr int kFeatures = 2 + kModalCoef;
