@@ -374,50 +374,77 @@ __global__ void k_vector_gather(const InstDesc *descs, int ninst, int total)
     d.R[I] = acc;
 }
 
-// Deterministic gather of element matrices into the lower band: one thread per band slot sums
-// the (p+1)^2 candidate elements that could hold both control points, in a fixed order (ey, then
-// ex, ascending); candidates outside the patch or not sharing both points are masked loads (0 from
-// beyond the buffer resource), so every thread issues the same straight-line loads.  Two thirds
-// to five sixths of the slots of a p = 3 band (16 to 48 elements a side) are reached by no
-// element; when the destination is known to hold 0 there (skip_zero) those threads store
-// nothing, which leaves the same bits as storing the 0 sum.
+// Deterministic gather of the element matrices into the band, by control-point pairs.  A wave of
+// 64 lanes takes 64 consecutive pairs (cpJ, cpI) of one instance: cpI = cpJ + di + mx dj with
+// dj in [0, P] and di in [-P, P] (di >= 0 when dj = 0) -- every control point within one
+// element's reach of cpJ that comes at or after it in the numbering, so every band slot any
+// element reaches belongs to exactly one pair (free dofs are numbered 3 cp + c in increasing order,
+// and those pairs are the lower triangle).  The lane sums the pair's 3 x 3 block over the elements
+// holding both points, in a fixed element order (rows ey, then columns ex), and stores the slots
+// of its free dofs.  Slots no element reaches are never written: the band must hold 0 there (a
+// fresh band, or one zeroed by the launcher).  Reads each stored element entry once.
 template <int P, typename T>
-__global__ __launch_bounds__(512) void k_band_gather(const InstDesc *descs, int ninst, long long total)
+__global__ __launch_bounds__(256) void k_band_gather(const InstDesc *descs, int ninst)
 {
-    constexpr int P1 = P + 1;
-    long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= total) return;
-    const InstDesc &d = descs[find_inst(ninst, gid, [&](int m) { return descs[m].slot0; })];
-    long long slot = gid - d.slot0;
-    const long long ld = (long long)d.bw + 1;
-    int J = (int)(slot / ld), dd = (int)(slot % ld), I = J + dd;
-    const bool row_ok = I < d.nfree;
-    const int gI = d.free2full[row_ok ? I : J], gJ = d.free2full[J];
-    const int cpI = gI / 3, cI = gI % 3, cpJ = gJ / 3, cJ = gJ % 3;
-    const int iI = cpI % d.mx, jI = cpI / d.mx, iJ = cpJ % d.mx, jJ = cpJ / d.mx;
+    constexpr int P1 = P + 1, NPAIR = P1 + P * (2 * P + 1);
+    const long long chunk = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const InstDesc &d = descs[find_inst(ninst, chunk, [&](int m) { return descs[m].chunk0; })];
+    const long long t = (chunk - d.chunk0) * 64 + (threadIdx.x & 63);
+    const int my = d.ny + P, ncp = d.mx * my;
+    if (t >= (long long)ncp * NPAIR) return;
+    const int cpJ = (int)(t / NPAIR), k = (int)(t % NPAIR);
+    const int dj = k < P1 ? 0 : 1 + (k - P1) / (2 * P + 1);
+    const int di = k < P1 ? k : (k - P1) % (2 * P + 1) - P;
+    const int iJ = cpJ % d.mx, jJ = cpJ / d.mx, iI = iJ + di, jI = jJ + dj;
+    if (iI < 0 || iI >= d.mx || jI >= my) return;
+    const int cpI = iI + d.mx * jI;
+    int fI[3], fJ[3];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        fI[c] = d.free_idx[3 * cpI + c];
+        fJ[c] = d.free_idx[3 * cpJ + c];
+    }
+#pragma unroll
+    for (int cJ = 0; cJ < 3; ++cJ)
+#pragma unroll
+        for (int cI = 0; cI < 3; ++cI) any |= fI[cI] >= 0 && fJ[cJ] >= 0 && !(cpI == cpJ && cI < cJ);
+    if (!any) return;
     const int ex0 = max(iI, iJ) - P, exhi = min(min(iI, iJ), d.nx - 1);
-    const int ey0 = max(jI, jJ) - P, eyhi = min(min(jI, jJ), d.ny - 1);
-    // no candidate element holds both points (ex0 > exhi or ey0 > eyhi: the index ranges cannot meet
-    // since ex0 + P >= exhi >= 0), or the row is past the end: the slot is 0 and already stored as 0
-    if (d.skip_zero && (!row_ok || ex0 > exhi || ey0 > eyhi)) return;
+    const int ey0 = jI - P, eyhi = min(jJ, d.ny - 1);
     const __amdgpu_buffer_rsrc_t rs = band_rsrc(d.Ke, (unsigned)((size_t)d.nel * kKeTile * sizeof(T)));
-    T v[P1][P1];
+    double acc[3][3];
+#pragma unroll
+    for (int cJ = 0; cJ < 3; ++cJ)
+#pragma unroll
+        for (int cI = 0; cI < 3; ++cI) acc[cJ][cI] = 0.0;
 #pragma unroll
     for (int ty = 0; ty < P1; ++ty)
 #pragma unroll
         for (int tx = 0; tx < P1; ++tx) {
             const int ex = ex0 + tx, ey = ey0 + ty;
-            const bool ok = row_ok && ex >= 0 && ex <= exhi && ey >= 0 && ey <= eyhi;
+            const bool ok = ex >= 0 && ex <= exhi && ey >= 0 && ey <= eyhi;
             const int A = (iI - ex) + P1 * (jI - ey), B = (iJ - ex) + P1 * (jJ - ey);
-            const unsigned off = (unsigned)(((size_t)(ex + d.nx * ey) * kKeTile + (3 * B + cJ) * kKeLd + 3 * A + cI) * sizeof(T));
-            v[ty][tx] = load_if<T>(ok, rs, off);
+            const size_t base = (size_t)(ex + d.nx * ey) * kKeTile;
+            T v[3][3];
+#pragma unroll
+            for (int cJ = 0; cJ < 3; ++cJ)
+#pragma unroll
+                for (int cI = 0; cI < 3; ++cI)
+                    v[cJ][cI] = load_if<T>(ok && !(cpI == cpJ && cI < cJ), rs,
+                                           (unsigned)((base + (size_t)(3 * B + cJ) * kKeLd + 3 * A + cI) * sizeof(T)));
+#pragma unroll
+            for (int cJ = 0; cJ < 3; ++cJ)
+#pragma unroll
+                for (int cI = 0; cI < 3; ++cI) acc[cJ][cI] += (double)v[cJ][cI];
         }
-    double acc = 0.0;
+    const long long ld = (long long)d.bw + 1;
 #pragma unroll
-    for (int ty = 0; ty < P1; ++ty)
+    for (int cJ = 0; cJ < 3; ++cJ)
 #pragma unroll
-        for (int tx = 0; tx < P1; ++tx) acc += (double)v[ty][tx];
-    ((T *)d.ab)[slot] = (T)acc;
+        for (int cI = 0; cI < 3; ++cI)
+            if (fI[cI] >= 0 && fJ[cJ] >= 0 && !(cpI == cpJ && cI < cJ))
+                ((T *)d.ab)[(long long)fJ[cJ] * ld + (fI[cI] - fJ[cJ])] = (T)acc[cJ][cI];
 }
 
 __global__ void k_point_loads(int npoint, int p, int mx, const int *free_idx, const double *pl_N, const int *pl_span,
@@ -455,7 +482,7 @@ ElemArgs host_elem_args(const Problem &pb)
 }
 
 struct Totals {
-    long long nel = 0, nfree = 0, ndof = 0, slots = 0;
+    long long nel = 0, nfree = 0, ndof = 0, slots = 0, chunks = 0;
 };
 
 // prefix offsets of a batch
@@ -466,7 +493,9 @@ Totals prefix(std::vector<InstDesc> &h)
         d.elem0 = (int)t.nel;
         d.dof0 = (int)t.nfree;
         d.full0 = (int)t.ndof;
-        d.slot0 = t.slots;
+        const long long npair = (long long)(d.nx + d.p) * (d.ny + d.p) * ((d.p + 1) + d.p * (2 * d.p + 1));
+        d.chunk0 = t.chunks;
+        t.chunks += (npair + 63) / 64;
         t.nel += d.nel;
         t.nfree += d.nfree;
         t.ndof += d.ndof;
@@ -581,14 +610,17 @@ int run_tangent(hipStream_t s, InstDesc *d_desc, std::vector<InstDesc> &h, bool 
         }
         ps.set_work(work);
     }
-    // variants: 128, 256 or 512 threads per block (one band slot per thread either way)
+    // bands that may hold values in slots no element reaches are cleared first
+    for (const auto &d : h)
+        if (!d.skip_zero) KSH_HIP(hipMemsetAsync(d.ab, 0, (size_t)d.nfree * (d.bw + 1) * (f64 ? 8 : 4), s));
+    // variants: 64, 128 or 256 threads per block (one control-point pair per thread either way)
     auto launch_gather = [&](int v) {
-        const unsigned bs = 128u << v;
-        const dim3 gs(nblk(t.slots, bs)), bb(bs);
-        if (p2 && f64) hipLaunchKernelGGL((k_band_gather<2, double>), gs, bb, 0, s, d_desc, n, t.slots);
-        else if (p2) hipLaunchKernelGGL((k_band_gather<2, float>), gs, bb, 0, s, d_desc, n, t.slots);
-        else if (f64) hipLaunchKernelGGL((k_band_gather<3, double>), gs, bb, 0, s, d_desc, n, t.slots);
-        else hipLaunchKernelGGL((k_band_gather<3, float>), gs, bb, 0, s, d_desc, n, t.slots);
+        const unsigned wpb = 1u << v, bs = 64u * wpb;
+        const dim3 gs((unsigned)((t.chunks + wpb - 1) / wpb)), bb(bs);
+        if (p2 && f64) hipLaunchKernelGGL((k_band_gather<2, double>), gs, bb, 0, s, d_desc, n);
+        else if (p2) hipLaunchKernelGGL((k_band_gather<2, float>), gs, bb, 0, s, d_desc, n);
+        else if (f64) hipLaunchKernelGGL((k_band_gather<3, double>), gs, bb, 0, s, d_desc, n);
+        else hipLaunchKernelGGL((k_band_gather<3, float>), gs, bb, 0, s, d_desc, n);
     };
     tune_launch(mass ? "mass_gather" : "band_gather", tune_bucket((double)t.slots) * 4 + (p2 ? 0 : 1) * 2 + (f64 ? 1 : 0), 3, 1,
                 s, launch_gather);

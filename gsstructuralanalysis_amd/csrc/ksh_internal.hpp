@@ -52,7 +52,7 @@ struct InstDesc {
     int dof0;               // first free dof
     int full0;              // first full dof
     int skip_zero;          // 1: band slots no element reaches already hold 0 in ab and are not stored
-    long long slot0;        // first band slot (n*(bw+1) per instance)
+    long long chunk0;       // band gather: first 64-thread chunk of this instance
     double t, rho, lambda;
     int mode, sprec;        // residual gather mode (0 sum, 1 ALM, 2 static); band storage precision
     const double *ctrl0, *tabx, *taby, *gpgeo, *F, *u_free;

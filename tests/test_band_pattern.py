@@ -1,12 +1,11 @@
 """Band slots that no element reaches (CPU, no GPU).
 
-The band gather (gsstructuralanalysis_amd/csrc/ksh_assembly.hip, k_band_gather) treats a slot
-(I, J) of the lower band as unreached when I is past the last free dof, or when the element
-index ranges of the two control points cannot meet (an element e spans control points e..e+p in
-each direction).  While a band is known to hold 0 in those slots the gather stores nothing
-there, so every slot that the oracle's assembly makes nonzero must be reached.  This restates
-the kernel's slot test and checks it against the oracle's band on small patches, and checks the
-share of unreached slots at the largest headline size, which the saving in stores rests on.
+The band gather (gsstructuralanalysis_amd/csrc/ksh_assembly.hip, k_band_gather) enumerates
+control-point pairs (cpJ, cpI = cpJ + di + mx dj), dj in [0, p], di in [-p, p] (di >= 0 when
+dj = 0), and writes the slots of their free dofs (cI >= cJ on the diagonal pair); it never writes
+any other slot, which must therefore hold 0.  This restates that enumeration and checks against the
+oracle's band on small patches that every slot it leaves out is 0 (in K and M), that no slot is
+written twice, and the share of slots left out at the largest headline size.
 """
 import numpy as np
 import pytest
@@ -15,18 +14,33 @@ from gsstructuralanalysis_amd import geometry as g
 from oracle.ok import Oracle
 
 
-def unreached(p, nx, ny, free, bw):
-    """Per band slot (column-major, J * (bw + 1) + (I - J)): True if no element holds both dofs."""
-    nfree = len(free)
-    mx = nx + p
-    J = np.repeat(np.arange(nfree), bw + 1)
-    I = J + np.tile(np.arange(bw + 1), nfree)
-    row_ok = I < nfree
-    cpI, cpJ = free[np.where(row_ok, I, J)] // 3, free[J] // 3
-    iI, jI, iJ, jJ = cpI % mx, cpI // mx, cpJ % mx, cpJ // mx
-    ex0, exhi = np.maximum(iI, iJ) - p, np.minimum(np.minimum(iI, iJ), nx - 1)
-    ey0, eyhi = np.maximum(jI, jJ) - p, np.minimum(np.minimum(jI, jJ), ny - 1)
-    return ~row_ok | (ex0 > exhi) | (ey0 > eyhi)
+def written(p, nx, ny, free_idx, nfree, bw):
+    """Per band slot (column-major, J * (bw + 1) + (I - J)): how many gather threads write it."""
+    mx, my = nx + p, ny + p
+    cnt = np.zeros(nfree * (bw + 1), dtype=np.int64)
+    pairs = [(0, di) for di in range(p + 1)] + [(dj, di) for dj in range(1, p + 1) for di in range(-p, p + 1)]
+    for cpJ in range(mx * my):
+        iJ, jJ = cpJ % mx, cpJ // mx
+        for dj, di in pairs:
+            iI, jI = iJ + di, jJ + dj
+            if iI < 0 or iI >= mx or jI >= my:
+                continue
+            cpI = iI + mx * jI
+            for cJ in range(3):
+                for cI in range(3):
+                    if cpI == cpJ and cI < cJ:
+                        continue
+                    I, J = free_idx[3 * cpI + cI], free_idx[3 * cpJ + cJ]
+                    if I >= 0 and J >= 0:
+                        assert 0 <= I - J <= bw
+                        cnt[J * (bw + 1) + (I - J)] += 1
+    return cnt
+
+
+def unreached(pt, o):
+    c = written(pt.p, pt.nx, pt.ny, o.free_idx, o.nfree, o.bw)
+    assert c.max() == 1  # no slot has two writers
+    return c == 0
 
 
 CASES = {
@@ -44,7 +58,7 @@ def test_unreached_slots_are_zero_in_the_oracle_band(name):
     u = np.random.default_rng(3).standard_normal(o.nfree) * 1e-2
     K = o.tangent_band(u)
     M = o.mass_band()
-    skip = unreached(pt.p, pt.nx, pt.ny, o.free, o.bw)
+    skip = unreached(pt, o)
     assert skip.shape == K.shape
     assert np.all(K[skip] == 0.0) and np.all(M[skip] == 0.0)
     # the test is not vacuous: it skips slots, and the slots it keeps are (nearly all) nonzero
@@ -55,5 +69,5 @@ def test_unreached_slots_are_zero_in_the_oracle_band(name):
 def test_unreached_share_at_the_largest_headline_size():
     pt = g.imperfect_roof_instance(0, sizes=(48,))
     o = Oracle(pt)
-    share = unreached(pt.p, pt.nx, pt.ny, o.free, o.bw).mean()
+    share = unreached(pt, o).mean()
     assert 0.8 < share < 0.9, share
