@@ -288,15 +288,16 @@ def main():
         tr_sps = [ShellProblem(g.imperfect_roof_instance(i, seed0=50261202)) for i in range(ntrain)]
         tr_len = np.array([g.roof_arc_length(s.nfree, C2_KAPPA) for s in tr_sps])
         tr, _ = run_alm_batch(tr_sps, C2_STEPS, tr_len, features=True, **kw, **forms["batched"])
-        X, Y = dataset([r["features"][:r["steps_done"]] for r in tr])
+        X, Y, grp = dataset([r["features"][:r["steps_done"]] for r in tr], with_groups=True)
         t_data = time.perf_counter() - t0
         net = LearnedPredictor(seed=1)
         t0 = time.perf_counter()
-        losses = net.fit(X, Y, epochs=args.epochs, batch=256, lr=1e-3)
+        losses = net.fit(X, Y, epochs=args.epochs, batch=256, lr=1e-3, groups=grp, val_frac=0.125)
         t_fit = time.perf_counter() - t0
         del tr_sps
         training = {"instances": ntrain, "seeds": "50261202 + i", "samples": int(len(X)), "epochs": args.epochs,
                     "loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1]),
+                    "validation": net.validation,
                     "data_seconds": round(t_data, 2), "train_seconds": round(t_fit, 2)}
         log(f"[bench] learned predictor: {len(X)} samples from {ntrain} roofs in {t_data:.1f} s; "
             f"{args.epochs} epochs in {t_fit:.1f} s, loss {losses[0]:.3e} -> {losses[-1]:.3e}")

@@ -126,14 +126,20 @@ __global__ __launch_bounds__(256) void k_mse(int count, float scale, const float
     if (threadIdx.x == 0) part[blockIdx.x] = s[0] * scale;
 }
 
-// db[n] = sum_m dZ[m][n] (fixed order over m)
-__global__ void k_colsum(int M, int N, const float *dZ, float *db)
+// db[n] = sum_m dZ[m][n]: a workgroup of 4 waves per 64 columns, lanes along the columns (coalesced
+// rows), wave w summing rows w, w + 4, ... in order, then the four partial sums in wave order
+// (a fixed order, so the result is reproducible)
+__global__ __launch_bounds__(256) void k_colsum(int M, int N, const float *dZ, float *db)
 {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
+    __shared__ float part[4][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = blockIdx.x * 64 + lane;
     float s = 0.f;
-    for (int m = 0; m < M; ++m) s += dZ[(size_t)m * N + n];
-    db[n] = s;
+    if (n < N)
+        for (int m = wave; m < M; m += 4) s += dZ[(size_t)m * N + n];
+    part[wave][lane] = s;
+    __syncthreads();
+    if (wave == 0 && n < N) db[n] = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
 }
 
 __global__ void k_adam(int count, float *w, const float *g, float *mo, float *ve, float lr, float b1, float b2, float eps,
@@ -245,7 +251,7 @@ struct Mlp {
             const int K = width[l], N = width[l + 1];
             // dW = H[l]^T dZ[l]   (K x N, reduction over the batch)
             if ((rc = gemm(K, N, B, H[l], 1, K, dZ[l], N, 1, nullptr, 0, gW(l), N, nullptr))) return rc;
-            hipLaunchKernelGGL(k_colsum, dim3(nb(N, 256)), dim3(256), 0, s, B, N, dZ[l], gb(l));
+            hipLaunchKernelGGL(k_colsum, dim3(nb(N, 64)), dim3(256), 0, s, B, N, dZ[l], gb(l));
             KSH_HIP(hipGetLastError());
             if (l > 0) {
                 // dH[l] = dZ[l] W^T, then through GELU of layer l-1

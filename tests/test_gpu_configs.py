@@ -1,0 +1,178 @@
+"""The workloads at their full sizes on the GPU.
+
+C0  static Newton, 16 x 16 degree-2 clamped plate, q = 1e2, 5 load steps, tol 1e-10, against the
+    oracle's restatement of gsStaticNewton under gsControlDisplacement (CPU, run here).
+C1  Crisfield on the 32 x 32 degree-3 roof, 80 steps, plain FP64 and mixed (FP32 factor, FGMRES in
+    FP64), against the oracle's path committed in tests/golden/c1_roof_path.json
+    (tests/golden/make_c1_roof.py): load factors, ||U||, centre deflection within 1e-7 of the
+    path's scale; negative-pivot counts equal at every step, corrector iterations too (FP64; the
+    mixed form's solves to 1e-6 may move one iteration in a step).
+C2  the 256-instance imperfection sweep, 60 steps each: the mixed form against the FP64 batch
+    (identical negative-pivot counts at every step, load factors within 1e-7), every size class
+    crosses a limit point.
+C3  the learned predictor trained on 256 x 60 solver-generated path points, evaluated on 64
+    held-out roofs against the tangent predictor and the extrapolation alone.
+Plus the FP32 pivot-count certificate with the shift inside the FP32 noise of an eigenvalue.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from gsstructuralanalysis_amd import _lib as L
+from gsstructuralanalysis_amd import geometry as g
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ALM = dict(method=0, max_iter=20, tol=1e-6, tol_f=1e-8, tol_u=1e-6, scaling=0.0, bifurcation_method=0, max_halvings=10)
+MIXED = dict(precision=L.FP32_IR, max_refine=4, refine_tol=1e-6)
+C2_KAPPA = 2.6
+
+
+def test_c0_static_newton_plate(ksh):
+    from gsstructuralanalysis_amd.problem import ShellProblem
+    from oracle.solvers import run_static_newton
+    pt = g.clamped_plate_problem(p=2, n=16, q=1e2)
+    print(f"[case] C0 static Newton clamped plate p=2 16x16 ({pt.nfree} free dofs)", flush=True)
+    ref = run_static_newton(pt, 5, 0.2, max_iter=25, tol=1e-10)
+    got = ShellProblem(pt).run_static(5, 0.2, max_iter=25, tol=1e-10, keep_U=True)
+    assert ref["steps_done"] == got["steps_done"] == 5
+    assert np.array_equal(got["lambda"], ref["lambda"])
+    assert np.array_equal(got["iterations"], ref["iterations"])
+    assert np.array_equal(got["negatives"], ref["negatives"])
+    scale = np.abs(ref["U"]).max()
+    assert np.abs(got["U"] - ref["U"]).max() <= 1e-7 * scale
+    # strongly nonlinear: deflections of more than the thickness, well below the linear theory's
+    # centre deflection 0.00126 q a^4 / D = 0.1376 (membrane stiffening)
+    assert pt.t < scale < 0.5 * 0.1376
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_c1_roof_path_against_oracle(form, ksh):
+    from gsstructuralanalysis_amd.problem import ShellProblem
+    with open(os.path.join(HERE, "golden", "c1_roof_path.json")) as f:
+        fx = json.load(f)
+    pt = g.roof_problem(p=3, n=32)
+    assert pt.nfree == fx["nfree"]
+    steps = fx["steps"]
+    print(f"[case] C1 roof p=3 32x32, {steps} steps, ds {fx['arc_length']:.4g}, {form}", flush=True)
+    extra = dict(precision=L.FP64) if form == "fp64" else MIXED
+    got = ShellProblem(pt).run_alm(steps, fx["arc_length"], probe_dof=fx["centre_dof"], **ALM, **extra)
+    assert got["steps_done"] == steps and got["halvings"] == fx["halvings"]
+    for key, ref in (("lambda", fx["lambda"]), ("unorm", fx["unorm"]), ("w", fx["w_centre"])):
+        ref = np.asarray(ref)
+        err = float(np.abs(got[key] - ref).max() / np.abs(ref).max())
+        print(f"[C1 {form}] {key}: max relative deviation {err:.2e}", flush=True)
+        assert err <= 1e-7, key
+    assert np.array_equal(got["negatives"], fx["negatives"])
+    assert max(fx["negatives"]) >= 1  # the path crosses the limit point
+    assert (got["residue_f"] < 1e-8).all()
+    its, ref_its = np.asarray(got["iterations"]), np.asarray(fx["iterations"])
+    print(f"[C1 {form}] corrector iterations {its.sum()} (oracle {ref_its.sum()})", flush=True)
+    if form == "fp64":
+        assert np.array_equal(its, ref_its)
+    else:  # solves to 1e-6: the Newton iterates differ slightly, the converged points do not
+        assert np.abs(its - ref_its).max() <= 1 and abs(int(its.sum()) - int(ref_its.sum())) <= 0.05 * ref_its.sum()
+
+
+def _c2(n=256):
+    from gsstructuralanalysis_amd.problem import ShellProblem
+    sps = [ShellProblem(g.imperfect_roof_instance(i)) for i in range(n)]
+    ds = np.array([g.roof_arc_length(s.nfree, C2_KAPPA) for s in sps])
+    return sps, ds
+
+
+def test_c2_batch_mixed_against_fp64(ksh):
+    from gsstructuralanalysis_amd.problem import run_alm_batch
+    sps, ds = _c2()
+    print("[case] C2 256 imperfect roofs x 60 steps: FP64 batch, then the mixed batch", flush=True)
+    r64, t64 = run_alm_batch(sps, 60, ds, precision=L.FP64, **ALM)
+    rmx, tmx = run_alm_batch(sps, 60, ds, **ALM, **MIXED)
+    print(f"[C2] FP64 {t64:.2f} s, mixed {tmx:.2f} s", flush=True)
+    crossed = {}
+    for sp, a, b in zip(sps, r64, rmx):
+        assert a["steps_done"] == b["steps_done"] == 60
+        assert np.array_equal(a["negatives"], b["negatives"])
+        assert np.abs(a["lambda"] - b["lambda"]).max() <= 1e-7 * np.abs(a["lambda"]).max()
+        assert (b["residue_f"] < 1e-8).all()
+        crossed.setdefault(sp.patch.nx, []).append(bool((a["negatives"] > 0).any()))
+    for n, v in sorted(crossed.items()):
+        print(f"[C2] n={n}: {sum(v)}/{len(v)} instances cross a limit point", flush=True)
+        assert sum(v) >= 0.5 * len(v)
+    assert sorted(crossed) == [16, 24, 32, 40, 48]
+
+
+def test_c3_learned_predictor_on_held_out_roofs(ksh):
+    from gsstructuralanalysis_amd.learned import LearnedPredictor, dataset
+    from gsstructuralanalysis_amd.problem import ShellProblem, run_alm_batch
+    train = [ShellProblem(g.imperfect_roof_instance(i, seed0=50261202)) for i in range(256)]
+    tds = np.array([g.roof_arc_length(s.nfree, C2_KAPPA) for s in train])
+    tr, _ = run_alm_batch(train, 60, tds, features=True, precision=L.FP64, **ALM)
+    X, Y, grp = dataset([r["features"][:r["steps_done"]] for r in tr], with_groups=True)
+    print(f"[case] C3: {len(X)} training samples from 256 x 60 paths", flush=True)
+    assert len(X) == 256 * (60 - 4)
+    del train
+    net = LearnedPredictor(seed=1)
+    net.fit(X, Y, epochs=400, batch=256, lr=1e-3, groups=grp, val_frac=0.125)
+    print(f"[C3] validation: {net.validation}", flush=True)
+    held = [ShellProblem(g.imperfect_roof_instance(i, seed0=30261202)) for i in range(64)]
+    hds = np.array([g.roof_arc_length(s.nfree, C2_KAPPA) for s in held])
+    arms = {}
+    for name, extra in (("tangent", {}), ("extrapolation", dict(learned=net, learned_mode=1)),
+                        ("learned", dict(learned=net))):
+        rs, secs = run_alm_batch(held, 60, hds, precision=L.FP64, **ALM, **extra)
+        arms[name] = rs
+        print(f"[C3 {name}] {secs:.2f} s, corrector iterations {sum(int(r['iterations'].sum()) for r in rs)}, "
+              f"factorizations {sum(r['jacobians'] for r in rs)}, learned steps {sum(r['learned_steps'] for r in rs)}, "
+              f"fallbacks {sum(r['fallbacks'] for r in rs)}", flush=True)
+    base = arms["tangent"]
+    for name in ("extrapolation", "learned"):
+        rs = arms[name]
+        used = sum(r["learned_steps"] for r in rs)
+        assert used > 0.8 * 56 * 64
+        assert sum(r["fallbacks"] for r in rs) <= 0.05 * used
+        for a, b in zip(base, rs):
+            assert a["steps_done"] == b["steps_done"] == 60
+            assert np.array_equal(a["negatives"], b["negatives"])
+            assert np.abs(a["lambda"] - b["lambda"]).max() <= 1e-6 * np.abs(a["lambda"]).max()
+        # the proposal saves the predictor's tangent and factorisation on every learned step
+        assert sum(r["jacobians"] for r in rs) < sum(r["jacobians"] for r in base)
+
+
+def test_fp32_count_certificate_near_an_eigenvalue(ksh):
+    """K - sigma M with sigma inside the FP32 noise of an eigenvalue of the pencil: the FP32 factor's
+    pivot count is not trustworthy there, its residual contraction says so, and the count comes
+    from an FP64 factor (the matrix's D is then the FP64 factor's D, bit for bit)."""
+    import scipy.linalg as sl
+    from gsstructuralanalysis_amd.problem import BandMatrix, ShellProblem
+    from oracle import ok
+    from oracle.ok import Oracle
+    pt = g.simply_supported_plate_problem(p=3, n=10, rho=1e3)
+    o = Oracle(pt)
+    sp = ShellProblem(pt)
+    Kb = o.tangent_band(np.zeros(o.nfree))
+    Mb = o.mass_band()
+    ev = sl.eigh(ok.band_to_dense(Kb, o.nfree, o.bw), ok.band_to_dense(Mb, o.nfree, o.bw), eigvals_only=True)
+    fired = 0
+    for k in (3, 7, 12):
+        for rel in (-1e-9, 1e-9):
+            sigma = ev[k] * (1 + rel)
+            ab = Kb - sigma * Mb
+            exact = int(np.sum(ev < sigma))
+            ref = ok.BandLDLT(ab, o.nfree, o.bw)
+            A64 = BandMatrix(sp, precision=L.FP64)
+            A64.upload(ab)
+            A64.factor()
+            n64, _ = A64.inertia()
+            A = BandMatrix(sp, precision=L.FP32_IR)
+            A.upload(ab)
+            A.factor()
+            neg, _ = A.inertia()
+            print(f"[case] sigma = lambda_{k} (1 {rel:+.0e}): exact {exact}, oracle {ref.negatives()[0]}, "
+                  f"FP64 {n64}, FP32 certified {neg}", flush=True)
+            assert neg == n64 == exact
+            if np.array_equal(A.vector_d(), A64.vector_d()):
+                fired += 1
+    assert fired >= 4  # the certificate sent (nearly) every one of these counts to FP64
