@@ -94,6 +94,7 @@ SIGNATURES = {
     "ksh_assemble_force": (C.c_int, [_VP, _dp]),
     "ksh_assemble_residual": (C.c_int, [_VP, _dp, C.c_double, C.c_int, _dp]),
     "ksh_assemble_residual_dev": (C.c_int, [_VP, _VP, C.c_double, C.c_int, _VP]),
+    "ksh_assemble_residual_dev_noalloc": (C.c_int, [_VP, _VP, C.c_double, C.c_int, _VP]),
     "ksh_matrix_create": (C.c_int, [_VP, C.c_int, C.POINTER(_VP)]),
     "ksh_matrix_destroy": (C.c_int, [_VP]),
     "ksh_matrix_dims": (C.c_int, [_VP, _i64p, _i64p, _ip]),

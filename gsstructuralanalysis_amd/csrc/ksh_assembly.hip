@@ -719,6 +719,22 @@ int launch_residual(Problem &pb, double lambda, int conv, double *R_dev)
     return run_residual(pb.stream, pb.d_desc, h, false);
 }
 
+// residual with caller-chosen partial-result storage (u_full: ndof, Hg: 15 ngp, Fe: 3 nb nel doubles)
+int run_residual_into(Problem &pb, const double *u_free_dev, double lambda, int conv, double *R_dev, double *u_full,
+                      double *Hg, double *Fe)
+{
+    InstDesc d = make_desc(pb);
+    d.u_free = u_free_dev;
+    d.u_full = u_full;
+    d.Hg = Hg;
+    d.Fe = Fe;
+    d.lambda = lambda;
+    d.mode = (conv == KSH_RES_ALM) ? 1 : 2;
+    d.R = R_dev;
+    std::vector<InstDesc> h = {d};
+    return run_residual(pb.stream, pb.d_desc, h, true);
+}
+
 int launch_external_force(Problem &pb)
 {
     ElemArgs ar = host_elem_args(pb);

@@ -101,6 +101,14 @@ int ksh_assemble_residual(ksh_problem *pb, const double *u_free, double lambda, 
                           double *R_out);
 int ksh_assemble_residual_dev(ksh_problem *pb, const double *u_free_dev, double lambda, int conv,
                               double *R_out_dev);
+/* No counterpart in the reference: ksh_assemble_residual_dev for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops).  Same arguments and result, bit for bit;
+ * its partial results go to one fixed 48 MiB device array of this library per GPU instead of the
+ * problem's workspace, and nothing is allocated or freed.  Problems whose partial results exceed
+ * the array (about 20k degree-3 elements) are rejected with KSH_OTHER_ERROR.  Calls that overlap
+ * on one GPU race on the array. */
+int ksh_assemble_residual_dev_noalloc(ksh_problem *pb, const double *u_free_dev, double lambda, int conv,
+                                      double *R_out_dev);
 
 /* ---- matrices -------------------------------------------------------------- */
 int ksh_matrix_create(ksh_problem *pb, int prec, ksh_matrix **out);
