@@ -72,11 +72,16 @@ __device__ __forceinline__ void diag_factor(double *blk, double *dval, double *d
 #pragma unroll
     for (int jj = 0; jj < kNS; ++jj) {
         const double cv = a[jj];  // A[r][jj]; lane kk's value is A[kk][jj]
-        const double djj = readlane_f64(cv, jj);
-        if (jj < w && bad == 0 && (djj == 0.0 || !isfinite(djj) || !isfinite((double)(T)djj))) bad = col0 + jj + 1;
-        const double l = cv / djj;
+        // the whole column first, into distinct scalar registers: the reads issue back to back and
+        // the reciprocal overlaps them, instead of a read / wait / multiply-add triple per entry
+        double col[kNS];
 #pragma unroll
-        for (int kk = jj + 1; kk < kNS; ++kk) a[kk] -= l * readlane_f64(cv, kk);
+        for (int kk = jj; kk < kNS; ++kk) col[kk] = readlane_f64(cv, kk);
+        const double djj = col[jj];
+        if (jj < w && bad == 0 && (djj == 0.0 || !isfinite(djj) || !isfinite((double)(T)djj))) bad = col0 + jj + 1;
+        const double l = cv * (1.0 / djj);
+#pragma unroll
+        for (int kk = jj + 1; kk < kNS; ++kk) a[kk] -= l * col[kk];
         if (r > jj) a[jj] = l;
         __builtin_amdgcn_sched_barrier(0);
     }
