@@ -3,7 +3,10 @@ dofs, band width 1166 on the free dofs), C0 plate with a 0.05 sin(pi x) sin(pi y
 10 lowest pairs (solvers/modal_shell_XML.cpp, solvers/buckling_shell_XML.cpp;
 src/gsEigenSolvers/gsModalSolver.h, gsBucklingSolver.h).
 
-The dense oracle cannot run at this size, so the checks are the ones the size allows:
+Checks:
+  * the 10 eigenvalues against ARPACK (scipy eigsh, shift-invert with the oracle's band LDL^T) on
+    the oracle's K, M, K_NL at the same size, committed in tests/golden/c4_eigen.json
+    (tests/golden/make_c4_eigen.py): relative error <= 1e-8 each, FP64 and FP32-factor forms;
   * Sturm counts: the number of negative pivots of K - sigma M (multi-workgroup FP64 factor) at
     sigma between consecutive computed eigenvalues is the number of eigenvalues below sigma
     (Sylvester), so none is missed and their order is right;
@@ -12,6 +15,8 @@ The dense oracle cannot run at this size, so the checks are the ones the size al
 The same solver is pinned against dense FP64 eigenvalues of the oracle's matrices at small sizes
 in test_gpu_eigen.py.
 """
+import json
+import os
 import time
 
 import numpy as np
@@ -21,6 +26,19 @@ from gsstructuralanalysis_amd import _lib as L
 from gsstructuralanalysis_amd import geometry as g
 
 pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _golden():
+    with open(os.path.join(HERE, "golden", "c4_eigen.json")) as f:
+        return json.load(f)
+
+
+def _check_golden(name, got, ref):
+    got, ref = np.sort(np.asarray(got)[:len(ref)]), np.sort(np.asarray(ref))
+    err = np.abs(got - ref) / np.abs(ref)
+    print(f"[c4] {name}: max relative deviation from ARPACK {err.max():.2e}", flush=True)
+    assert err.max() <= 1e-8, (got, ref)
 
 
 def _z0(x, y):
@@ -72,6 +90,9 @@ def test_large_plate_modal(plate):
     lam = r64["values"]
     assert np.all(np.diff(lam) >= 0) and lam[0] > 0
     assert np.abs(r32["values"] - lam[:10]).max() <= 1e-8 * np.abs(lam[:10]).max()
+    fx = _golden()
+    _check_golden("modal FP64", lam[:10], fx["modal_omega2"])
+    _check_golden("modal FP32 factor", r32["values"], fx["modal_omega2"])
     z = np.zeros(n)
     K = sp.tangent(z)
     M = sp.mass()
@@ -105,6 +126,9 @@ def test_large_plate_buckling(plate):
     assert r64["converged"] == 10 and r32["converged"] == 10
     lam = r64["values"]
     assert np.abs(r32["values"] - lam).max() <= 1e-8 * np.abs(lam).max()
+    fx = _golden()
+    _check_golden("buckling FP64", lam, fx["buckling_mu"])
+    _check_golden("buckling FP32 factor", r32["values"], fx["buckling_mu"])
     z = np.zeros(n)
     KL = sp.tangent(z)
     KL.factor()
