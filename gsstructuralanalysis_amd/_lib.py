@@ -42,7 +42,10 @@ class AlmOptions(C.Structure):
                 ("adaptive", C.c_int), ("adaptive_iterations", C.c_int), ("max_refine", C.c_int),
                 ("refine_tol", C.c_double), ("max_halvings", C.c_int), ("verbose", C.c_int),
                 ("inertia_guard", C.c_double), ("learned", C.c_void_p), ("refine_accept", C.c_double),
-                ("learned_mode", C.c_int), ("lengths", _dp)]
+                ("learned_mode", C.c_int), ("lengths", _dp), ("singular_point", C.c_int),
+                ("sp_test_tol", C.c_double), ("sp_test_it", C.c_int), ("sp_tol_e", C.c_double),
+                ("sp_tol_b", C.c_double), ("perturbation", C.c_double), ("sp_fail", C.c_int),
+                ("length_after_sp", C.c_double)]
 
 
 class AlmResult(C.Structure):
@@ -51,7 +54,7 @@ class AlmResult(C.Structure):
                 ("iterations", _ip), ("residue_f", _dp), ("length", _dp), ("U", _dp), ("w", _dp),
                 ("probe_dof", C.c_int), ("wall_seconds", C.c_double), ("phi", C.c_double),
                 ("jacobians", C.c_int64), ("residuals", C.c_int64), ("solves", C.c_int64),
-                ("features", _dp), ("learned_steps", C.c_int64), ("fallbacks", C.c_int64)]
+                ("features", _dp), ("learned_steps", C.c_int64), ("fallbacks", C.c_int64), ("singular", _ip)]
 
 
 class StaticOptions(C.Structure):

@@ -74,6 +74,16 @@ int BatchDevice::tangent(const std::vector<TangentJob> &jobs)
 {
     if (jobs.empty()) return KSH_SUCCESS;
     int rc;
+    if ((rc = assemble(jobs))) return rc;
+    std::vector<Matrix *> ms;
+    for (const TangentJob &j : jobs) ms.push_back(j.K);
+    return factor_matrices(ms);
+}
+
+int BatchDevice::assemble(const std::vector<TangentJob> &jobs)
+{
+    if (jobs.empty()) return KSH_SUCCESS;
+    int rc;
     if ((rc = ensure_desc(jobs.size()))) return rc;
     for (int sp : {KSH_FP64, KSH_FP32}) {  // one launch per storage precision (normally one group)
         std::vector<InstDesc> h;
@@ -88,7 +98,6 @@ int BatchDevice::tangent(const std::vector<TangentJob> &jobs)
         }
         if (!h.empty() && (rc = batch_tangent(m_stream, (InstDesc *)m_desc.p, h))) return rc;
     }
-    std::vector<Matrix *> ms;
     for (const TangentJob &j : jobs) {
         Matrix &K = *j.K;
         K.ab_clean = true;
@@ -97,9 +106,43 @@ int BatchDevice::tangent(const std::vector<TangentJob> &jobs)
             K.has_u = true;
         }
         ++K.version;
-        ms.push_back(&K);
     }
-    return factor_matrices(ms);
+    return KSH_SUCCESS;
+}
+
+int BatchDevice::products(std::vector<ProductJob> &jobs, std::vector<double> &rr)
+{
+    rr.assign(jobs.size(), 0.0);
+    if (jobs.empty()) return KSH_SUCCESS;
+    const size_t nj = jobs.size();
+    int rc;
+    if ((rc = ensure_desc(nj))) return rc;
+    size_t ptot = 0, nmax = 0;
+    for (const ProductJob &q : jobs) {
+        if (q.x == q.r) return fail(KSH_OTHER_ERROR, "product needs distinct x and r");
+        ptot += 2 * (((size_t)q.K->n + 63) / 64);
+        nmax = std::max(nmax, (size_t)q.K->n);
+    }
+    if ((rc = m_ppart.ensure(ptot * sizeof(double)))) return rc;
+    if (nmax > m_zero_n) {
+        if ((rc = m_zero.ensure(nmax * sizeof(double)))) return rc;
+        KSH_HIP(hipMemsetAsync(m_zero.p, 0, nmax * sizeof(double), m_stream));
+        m_zero_n = nmax;
+    }
+    std::vector<RefDesc> h(nj);
+    size_t po = 0;
+    for (size_t i = 0; i < nj; ++i) {
+        const Matrix &K = *jobs[i].K;
+        h[i] = RefDesc{K.ab, K.sprec, K.n, K.bw, 1, jobs[i].b ? jobs[i].b : (const double *)m_zero.p,
+                       const_cast<double *>(jobs[i].x), jobs[i].r, (double *)m_norms.p + 2 * i, (double *)m_ppart.p + po, 0, 0};
+        po += 2 * (((size_t)K.n + 63) / 64);
+    }
+    if ((rc = refine_residual_batch(m_stream, (RefDesc *)m_rdesc.p, h))) return rc;
+    std::vector<double> nr(2 * nj);
+    KSH_HIP(hipMemcpyAsync(nr.data(), m_norms.p, sizeof(double) * nr.size(), hipMemcpyDeviceToHost, m_stream));
+    KSH_HIP(hipStreamSynchronize(m_stream));
+    for (size_t i = 0; i < nj; ++i) rr[i] = nr[2 * i];
+    return KSH_SUCCESS;
 }
 
 int BatchDevice::factor(const std::vector<Matrix *> &ms)

@@ -29,6 +29,14 @@ struct ResidualJob {
     double *R;          // result (device)
 };
 
+// r = b - K x with the assembled values of K (b = nullptr: the zero vector, r = -K x)
+struct ProductJob {
+    const Matrix *K;
+    const double *x;    // device
+    const double *b;    // device or nullptr
+    double *r;          // device, distinct from x
+};
+
 struct SolveJob {
     Matrix *K;
     const double *b;    // right-hand side (device, not overwritten)
@@ -52,6 +60,10 @@ public:
     // zero / non-finite pivot are redone in FP64).  Afterwards every K holds factored_version ==
     // version, info, fstats and the certificate state of its pivot count.
     int tangent(const std::vector<TangentJob> &jobs);
+    // K(u) of every job without a factorization (the factor, if any, becomes stale)
+    int assemble(const std::vector<TangentJob> &jobs);
+    // r = b - K x of every job; rr[i] = ||r||^2
+    int products(std::vector<ProductJob> &jobs, std::vector<double> &rr);
     int factor(const std::vector<Matrix *> &ms);
     int residual(const std::vector<ResidualJob> &jobs);
     // x = K^-1 b with refinement against the assembled K; FP32 solves whose refinement stalls are
@@ -89,7 +101,7 @@ private:
     void *m_hearly = nullptr;  // pinned: info + stats of the early factors
     size_t m_hearly_bytes = 0;
     DevScratch m_desc, m_bdesc, m_cdesc, m_rdesc, m_stats, m_info, m_norms, m_part, m_r, m_dotdesc, m_vecdesc, m_dotout;
-    DevScratch m_zero;
+    DevScratch m_zero, m_ppart;
     size_t m_zero_n = 0;
     long long m_refactors = 0;
 };

@@ -17,6 +17,11 @@
 // quadratic constraint with its root selection and complex-root fallbacks, the Riks linearised
 // constraint, gsStaticNewton's convergence test, and benchmark_Roof's driver loop (halve the arc
 // length and retry from the last converged point when a step does not converge).
+// Singular points (gsALMBase.hpp:422-856): an instance whose stability changes in a step leaves
+// the step loop for a small state machine -- the singular-point test (power iterations), the
+// optional bisection on the pivot count (ordinary steps of the same engine), the extended system
+// for the critical point and the branch switch -- advanced one stage per round, batched with the
+// other instances at the same stage, while the rest of the batch goes on stepping.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -47,7 +52,13 @@ double now_s()
 int sgn(double v) { return (0.0 < v) - (v < 0.0); }  // gsALMHelper.h:21-25
 
 // device vectors of one path instance (slices of one allocation)
-enum VecId { V_U, V_DU, V_DUOLD, V_DU_STEP, V_UT, V_UB, V_R, V_NR, V_X, V_UH1, V_UH2, V_UPREV, V_T1, V_T2, V_COUNT };
+enum VecId {
+    V_U, V_DU, V_DUOLD, V_DU_STEP, V_UT, V_UB, V_R, V_NR, V_X, V_UH1, V_UH2, V_UPREV, V_T1, V_T2, V_BASE_COUNT,
+    // singular points (allocated when they are handled): the critical mode V and its increment,
+    // W = V + DV, the driver's previous point, the bisection's last point, the extended system's
+    // mode solves, two products and the perturbed point
+    V_SPV = V_BASE_COUNT, V_SPDV, V_SPW, V_SPUOLD, V_SPBU, V_SPVT, V_SPVB, V_SPP, V_SPQ, V_SPY, V_COUNT
+};
 
 struct Slab {
     double *base = nullptr;
@@ -66,7 +77,10 @@ struct MatrixOwner {
     }
 };
 
-enum class Phase { Start, Correct, Done };
+enum class Phase { Start, Correct, Singular, Done };
+
+// stages of computeSingularPoint (+ switchBranch) for one instance
+enum class Sp { None, Test, BisectStart, BisectStep, BisectEval, BisectVector, ExtStart, ExtIter, Finish };
 
 constexpr int kHist = 4;  // converged states the learned predictor reads
 
@@ -100,6 +114,18 @@ struct Inst {
     double unorm_new = 0, DUDU = 0, last_arc = 0;
     int last_fail = KSH_SUCCESS;
     int64_t jac = 0, resid = 0, solves = 0;
+    // singular point: stage, K(U + eps W) of the extended system, the driver's previous load
+    // factor, the bisection's reference pivot / count / last load factor / saved arc length, the
+    // extended system's load increment, iteration counters, outcomes
+    Sp sp = Sp::None;
+    Matrix *Keps = nullptr;
+    double sp_Lold = 0, sp_Lbu = 0, sp_ref = 0, sp_arc = 0, sp_DL = 0;
+    int64_t sp_fa = 0;
+    int sp_k = 0, sp_kind = 0;
+    bool sp_test = false, sp_conv = false, sp_step_ok = false;
+    // what the step's record shows (a singular point is recorded after its procedure)
+    int rec_it = 0;
+    double rec_resF = 0, rec_arc = 0;
 };
 
 class AlmEngine {
@@ -158,6 +184,16 @@ private:
             return fail(KSH_OTHER_ERROR, "quasi-Newton iterations are not supported by the path engine");
         if (o.bifurcation_method != -1 && o.bifurcation_method != 0)
             return fail(KSH_OTHER_ERROR, "BifurcationMethod: only -1 (nothing) and 0 (determinant) are supported");
+        if (o.singular_point < 0 || o.singular_point > 2) return fail(KSH_OTHER_ERROR, "singular_point: 0, 1 or 2");
+        if (o.singular_point) {
+            if (o.sp_test_tol < 0 || o.sp_test_it < 0 || o.sp_tol_e < 0 || o.sp_tol_b < 0 || o.perturbation < 0 ||
+                o.length_after_sp < 0 || (o.sp_fail != 0 && o.sp_fail != 1))
+                return fail(KSH_OTHER_ERROR, "bad singular-point options");
+            m_sp_test_tol = o.sp_test_tol > 0 ? o.sp_test_tol : 1e-6;
+            m_sp_test_it = o.sp_test_it > 0 ? o.sp_test_it : 5;
+            m_sp_tol_e = o.sp_tol_e > 0 ? o.sp_tol_e : 1e-10;
+            m_sp_tau = o.perturbation > 0 ? o.perturbation : 1e3;
+        }
         if (o.learned) {
             if (o.method != 0) return fail(KSH_OTHER_ERROR, "the learned predictor serves the Crisfield method only");
             int in = 0, out = 0, rc;
@@ -172,19 +208,30 @@ private:
     int setup(ksh_problem *const *pbs, int count, ksh_alm_result *res)
     {
         m_in.resize(count);
-        size_t tot = 0;
-        for (int i = 0; i < count; ++i) tot += (size_t)pbs[i]->nfree;
-        const size_t bytes = (tot * V_COUNT + (size_t)count * kModalCoef) * sizeof(double);
+        size_t tot = 0, nmax = 0;
+        for (int i = 0; i < count; ++i) {
+            tot += (size_t)pbs[i]->nfree;
+            nmax = std::max(nmax, (size_t)pbs[i]->nfree);
+        }
+        const int nvec = m_o.singular_point ? V_COUNT : V_BASE_COUNT;
+        const size_t nones = m_o.singular_point ? nmax : 0;
+        const size_t bytes = (tot * nvec + (size_t)count * kModalCoef + nones) * sizeof(double);
         KSH_HIP(hipMalloc(&m_slab.base, bytes));
         KSH_HIP(hipMemsetAsync(m_slab.base, 0, bytes, m_dev.stream()));
         double *p = m_slab.base;
-        m_coef = p + tot * V_COUNT;
+        m_coef = p + tot * nvec;
+        if (nones) {  // the power iterations' start vector (gsALMBase.hpp:485, 822)
+            m_ones = m_coef + (size_t)count * kModalCoef;
+            std::vector<double> one(nones, 1.0);
+            KSH_HIP(hipMemcpyAsync(m_ones, one.data(), nones * sizeof(double), hipMemcpyHostToDevice, m_dev.stream()));
+            KSH_HIP(hipStreamSynchronize(m_dev.stream()));
+        }
         for (int i = 0; i < count; ++i) {
             Inst &I = m_in[i];
             I.pb = pbs[i];
             I.n = pbs[i]->nfree;
             I.res = &res[i];
-            for (int q = 0; q < V_COUNT; ++q) {
+            for (int q = 0; q < nvec; ++q) {
                 I.v[q] = p;
                 p += I.n;
             }
@@ -267,7 +314,8 @@ private:
 
     bool learned_usable(const Inst &I) const
     {
-        return m_o.learned && !I.learned_off && (int)I.hist.size() >= kHist && !I.lens.empty() && I.lens.back() == I.arc;
+        return m_o.learned && !I.learned_off && I.sp == Sp::None && (int)I.hist.size() >= kHist && !I.lens.empty() &&
+               I.lens.back() == I.arc;
     }
 
     void begin_step(Inst &I)
@@ -279,9 +327,13 @@ private:
         I.learned_step = false;
     }
 
-    int round(std::vector<Inst *> &live)
+    int round(std::vector<Inst *> &all)
     {
         int rc;
+        // instances at a singular point advance one stage of its procedure
+        std::vector<Inst *> live, sing;
+        for (Inst *I : all) (I->ph == Phase::Singular ? sing : live).push_back(I);
+        if (!sing.empty() && (rc = singular_round(sing))) return rc;
         // ---- A: learned proposals ------------------------------------------------------------
         std::vector<Inst *> lp;
         for (Inst *I : live)
@@ -485,6 +537,7 @@ private:
                 else riks_predictor(I, g, d);
             } else {
                 count_pivots(I);  // the counted tangent is the one of this iteration (gsALMBase.hpp:386-390)
+                I.stability = I.indicator < 0 ? 1 : -1;
                 if (m_o.method == 0) {
                     if (!crisfield_corrector(I, g, d)) {
                         cx.push_back(&I);
@@ -735,6 +788,12 @@ private:
     // point (benchmarks/benchmark_Roof.cpp:423-433)
     void fail_step(Inst &I, int code)
     {
+        if (I.sp == Sp::BisectStep) {  // a bisection step that failed: gsALMBase.hpp:783-795
+            I.ph = Phase::Singular;
+            I.sp = Sp::BisectEval;
+            I.sp_step_ok = false;
+            return;
+        }
         if (I.learned_step) {
             ++I.res->fallbacks;
             begin_step(I);
@@ -756,7 +815,8 @@ private:
     }
 
     // converged: gsALMCrisfield / gsALMRiks::iterationFinish, the stability of the counted
-    // tangent, and the driver's bookkeeping
+    // tangent, and the driver's bookkeeping (or the singular-point procedure when the stability
+    // changed, or the next stage of a bisection)
     int complete_steps(std::vector<Inst *> &conv)
     {
         int rc;
@@ -786,6 +846,7 @@ private:
             if (I->ph == Phase::Done) continue;
             VecDesc d;
             d.n = I->n;
+            if (m_o.singular_point && I->sp == Sp::None) d.add(I->v[V_SPUOLD], 1.0, I->v[V_U]);  // the driver's Uold
             if (m_o.learned) {
                 d.add(I->v[V_UH2], 1.0, I->v[V_UH1]);
                 d.add(I->v[V_UH1], 1.0, I->v[V_U]);
@@ -797,10 +858,66 @@ private:
             fin.push_back(I);
         }
         if ((rc = m_dev.vecops(vd))) return rc;
+        std::vector<Inst *> rec;
+        for (Inst *I : fin) {
+            const double arc_used = I->arc;
+            I->last_arc = arc_used;
+            if (m_o.method == 1) {
+                I->Lprev = I->L;
+                I->prev_dnorm = std::sqrt(I->DUDU);
+            }
+            I->Lh2 = I->Lh1;
+            I->Lh1 = I->L;
+            if (I->sp == Sp::None) I->sp_Lold = I->L;  // the driver's Lold
+            I->L += I->DL;
+            if (m_o.method == 0 && m_o.angle_method == 0) {
+                I->DLold = I->DL;
+                I->prev_zero = I->DUDU == 0 && I->DL == 0;
+            } else if (m_o.method == 0) {
+                I->prev_zero = false;
+            }
+            I->stability = I->indicator < 0 ? 1 : -1;
+            // gsALMBase::_step after convergence (gsALMBase.hpp:396-403): adaptive length (off
+            // inside a bisection, gsALMBase.hpp:763-764)
+            if (m_o.adaptive && I->sp != Sp::BisectStep) {
+                double fac = (double)m_o.adaptive_iterations / I->it;
+                fac = std::min(2.0, std::max(0.5, fac));
+                I->arc_prev = I->arc;
+                I->arc *= fac;
+            } else {
+                I->arc_prev = I->arc;
+            }
+            if (I->sp == Sp::BisectStep) {  // gsALMBase.hpp:786-800: the bisection evaluates the point
+                I->ph = Phase::Singular;
+                I->sp = Sp::BisectEval;
+                I->sp_step_ok = true;
+                continue;
+            }
+            I->rec_it = I->it;
+            I->rec_resF = I->residueF;
+            I->rec_arc = arc_used;
+            // the drivers: computeStability(false) (the same counted tangent), stabilityChange()
+            if (m_o.singular_point && I->stability * I->stability_prev < 0) {
+                if (m_o.verbose)
+                    std::printf("[ksh path] step %d: stability changes at lambda %.10e, singular point\n", I->k, I->L);
+                I->ph = Phase::Singular;
+                I->sp = Sp::Test;
+                continue;
+            }
+            rec.push_back(I);
+        }
+        return record_steps(rec);
+    }
+
+    // the driver's record of converged points (Uold, Lold) and its arc-length reset
+    int record_steps(std::vector<Inst *> &rec)
+    {
+        int rc;
+        if (rec.empty()) return KSH_SUCCESS;
         // features of the new states (learned history, or requested by the caller)
         std::vector<FeatDesc> fd;
         std::vector<Inst *> fi;
-        for (Inst *I : fin)
+        for (Inst *I : rec)
             if (m_o.learned || I->res->features) {
                 FeatDesc f;
                 f.ncp = I->pb->ncp;
@@ -808,7 +925,7 @@ private:
                 f.P = I->pb->modal_P;
                 f.zfree = I->pb->modal_zfree;
                 f.U = I->v[V_U];
-                f.lambda = I->L + I->DL;
+                f.lambda = I->L;
                 fd.push_back(f);
                 fi.push_back(I);
             }
@@ -820,51 +937,32 @@ private:
             KSH_HIP(hipMemcpyAsync(feats.data(), m_feat.p, feats.size() * sizeof(double), hipMemcpyDeviceToHost, m_dev.stream()));
         }
         if ((rc = m_dev.synchronize())) return rc;
-        for (Inst *I : fin) {
+        for (Inst *I : rec) {
             ksh_alm_result &r = *I->res;
             const int k = I->k;
-            const double arc_used = I->arc;
-            I->last_arc = arc_used;
-            if (m_o.method == 1) {
-                I->Lprev = I->L;
-                I->prev_dnorm = std::sqrt(I->DUDU);
-            }
-            I->Lh2 = I->Lh1;
-            I->Lh1 = I->L;
-            I->L += I->DL;
-            if (m_o.method == 0 && m_o.angle_method == 0) {
-                I->DLold = I->DL;
-                I->prev_zero = I->DUDU == 0 && I->DL == 0;
-            } else if (m_o.method == 0) {
-                I->prev_zero = false;
-            }
-            I->stability = I->indicator < 0 ? 1 : -1;
-            // gsALMBase::_step after convergence (gsALMBase.hpp:396-403): adaptive length
-            if (m_o.adaptive) {
-                double fac = (double)m_o.adaptive_iterations / I->it;
-                fac = std::min(2.0, std::max(0.5, fac));
-                I->arc_prev = I->arc;
-                I->arc *= fac;
-            } else {
-                I->arc_prev = I->arc;
-            }
             if (r.lambda) r.lambda[k] = I->L;
             if (r.unorm) r.unorm[k] = I->unorm_new;
             if (r.indicator) r.indicator[k] = I->indicator;
             if (r.negatives) r.negatives[k] = I->negatives;
-            if (r.iterations) r.iterations[k] = I->it;
-            if (r.residue_f) r.residue_f[k] = I->residueF;
-            if (r.length) r.length[k] = arc_used;
+            if (r.iterations) r.iterations[k] = I->rec_it;
+            if (r.residue_f) r.residue_f[k] = I->rec_resF;
+            if (r.length) r.length[k] = I->rec_arc;
+            if (r.singular) r.singular[k] = I->sp_kind;
             if (r.U) KSH_HIP(hipMemcpy(r.U + (size_t)k * I->n, I->v[V_U], sizeof(double) * I->n, hipMemcpyDeviceToHost));
             if (r.w && r.probe_dof >= 0 && r.probe_dof < I->n)
-                KSH_HIP(hipMemcpy(&r.w[k], I->v[V_U] + r.probe_dof, sizeof(double), hipMemcpyDeviceToHost));
+                KSH_HIP(hipMemcpy(&r.w[k], I->v[V_U] + r.probe_dof, sizeof(double) * 1, hipMemcpyDeviceToHost));
             if (m_o.verbose)
-                std::printf("[ksh path] step %d: lambda %.10e |U| %.6e iterations %d negatives %lld length %.3e\n", k,
-                            I->L, I->unorm_new, I->it, (long long)I->negatives, arc_used);
+                std::printf("[ksh path] step %d: lambda %.10e |U| %.6e iterations %d negatives %lld length %.3e%s\n", k,
+                            I->L, I->unorm_new, I->rec_it, (long long)I->negatives, I->rec_arc,
+                            I->sp_kind == 1 ? " (limit point)" : I->sp_kind >= 2 ? " (bifurcation point)" : "");
             // the driver: arc length back to its start value after a step that was not halved
             if (!I->bisected) I->arc = I->arc_prev = I->arc0;
             I->bisected = false;
             I->learned_off = false;
+            if (I->sp_kind) {  // the path has a kink here: the learned history starts again
+                I->hist.clear();
+                I->lens.clear();
+            }
             I->k = k + 1;
             I->ph = I->k >= m_o.nsteps ? Phase::Done : Phase::Start;
         }
@@ -873,16 +971,441 @@ private:
             std::array<double, kFeatures> f;
             std::copy(feats.begin() + i * kFeatures, feats.begin() + (i + 1) * kFeatures, f.begin());
             if (I.res->features) std::copy(f.begin(), f.end(), I.res->features + (size_t)(I.k - 1) * kFeatures);
-            if (!I.lens.empty() && I.lens.back() != I.last_arc) {  // uniform steps only
+            if (!I.lens.empty() && I.lens.back() != I.rec_arc) {  // uniform steps only
                 I.hist.clear();
                 I.lens.clear();
             }
             I.hist.push_back(f);
-            I.lens.push_back(I.last_arc);
+            I.lens.push_back(I.rec_arc);
             while ((int)I.hist.size() > kHist) I.hist.pop_front();
             while ((int)I.lens.size() > kHist) I.lens.pop_front();
         }
+        for (Inst *I : rec) I->sp_kind = 0;
         return KSH_SUCCESS;
+    }
+
+    // ---- singular points -------------------------------------------------------------------------
+    // computeSingularPoint(Uold, Lold) of the drivers (gsALMBase.hpp:422-446), then switchBranch
+    // (gsALMBase.hpp:843-856) with singular_point = 2.  One stage per round for each instance.
+    int singular_round(std::vector<Inst *> &sing)
+    {
+        int rc;
+        auto stage = [&](Sp s) {
+            std::vector<Inst *> g;
+            for (Inst *I : sing)
+                if (I->ph == Phase::Singular && I->sp == s) g.push_back(I);
+            return g;
+        };
+        std::vector<Inst *> g;
+        if (!(g = stage(Sp::Test)).empty() && (rc = sp_test(g))) return rc;
+        if (!(g = stage(Sp::BisectStart)).empty() && (rc = sp_bisect_start(g))) return rc;
+        if (!(g = stage(Sp::BisectEval)).empty() && (rc = sp_bisect_eval(g))) return rc;
+        if (!(g = stage(Sp::BisectVector)).empty() && (rc = sp_bisect_vector(g))) return rc;
+        if (!(g = stage(Sp::ExtStart)).empty() && (rc = sp_ext_start(g))) return rc;
+        if (!(g = stage(Sp::ExtIter)).empty() && (rc = sp_ext_iter(g))) return rc;
+        if (!(g = stage(Sp::Finish)).empty() && (rc = sp_finish(g))) return rc;
+        return KSH_SUCCESS;
+    }
+
+    // solves of every job; a failed one ends its instance's path
+    int sp_solve(std::vector<SolveJob> &sj)
+    {
+        int rc;
+        if ((rc = m_dev.solve(sj))) return rc;
+        for (const SolveJob &q : sj) {
+            Inst *I = static_cast<Inst *>(q.tag);
+            ++I->solves;
+            if (q.rc != KSH_SUCCESS && I->ph != Phase::Done) {
+                I->status = q.rc;
+                I->ph = Phase::Done;
+            }
+        }
+        return KSH_SUCCESS;
+    }
+
+    // tangent + factor of each instance's K at the given vector; a zero or non-finite pivot ends
+    // the path (factorizeMatrix throws 3)
+    int sp_tangent(std::vector<Inst *> &g, VecId at)
+    {
+        std::vector<TangentJob> tj;
+        for (Inst *I : g) {
+            tj.push_back(TangentJob{I->pb, I->K, I->v[at]});
+            ++I->jac;
+        }
+        int rc;
+        if ((rc = m_dev.tangent(tj))) return rc;
+        for (Inst *I : g)
+            if (I->K->info) {
+                I->status = fail(KSH_SOLVER_ERROR, "LDL^T at the singular point: zero or non-finite pivot");
+                I->ph = Phase::Done;
+            }
+        return KSH_SUCCESS;
+    }
+
+    static void keep_live(std::vector<Inst *> &g)
+    {
+        std::vector<Inst *> ok;
+        for (Inst *I : g)
+            if (I->ph != Phase::Done) ok.push_back(I);
+        g.swap(ok);
+    }
+
+    // V <- K^-1 V / ||K^-1 V|| with each instance's current factor, from the normalised ones
+    // vector; `stop`: the bisection's early exit ||V - Vold|| < Tol (gsALMBase.hpp:822-837)
+    int sp_power(std::vector<Inst *> &g, int its, bool stop)
+    {
+        int rc;
+        std::vector<VecDesc> vd;
+        for (Inst *I : g) {
+            VecDesc d;
+            d.n = I->n;
+            d.add(I->v[V_SPV], 1.0 / std::sqrt((double)I->n), m_ones);
+            if (stop) d.add(I->v[V_SPQ], 0.0, m_ones);
+            vd.push_back(d);
+        }
+        if ((rc = m_dev.vecops(vd))) return rc;
+        std::vector<Inst *> act = g;
+        for (int it = 0; it < its && !act.empty(); ++it) {
+            std::vector<SolveJob> sj;
+            for (Inst *I : act) sj.push_back(SolveJob{I->K, I->v[V_SPV], I->v[V_T1], m_o.max_refine, m_o.refine_tol, KSH_SUCCESS, I});
+            if ((rc = sp_solve(sj))) return rc;
+            keep_live(act);
+            std::vector<DotDesc> dd;
+            for (Inst *I : act) dd.push_back(dot1(*I, I->v[V_T1], I->v[V_T1]));
+            std::vector<double> dv;
+            if ((rc = m_dev.dots(dd, dv))) return rc;
+            vd.clear();
+            for (size_t i = 0; i < act.size(); ++i) {
+                Inst &I = *act[i];
+                VecDesc d;
+                d.n = I.n;
+                d.add(I.v[V_SPV], 1.0 / std::sqrt(dv[i]), I.v[V_T1]);
+                if (stop) d.add(I.v[V_SPP], 1.0, I.v[V_SPV], -1.0, I.v[V_SPQ]);
+                vd.push_back(d);
+            }
+            if ((rc = m_dev.vecops(vd))) return rc;
+            if (!stop) continue;
+            dd.clear();
+            for (Inst *I : act) dd.push_back(dot1(*I, I->v[V_SPP], I->v[V_SPP]));
+            if ((rc = m_dev.dots(dd, dv))) return rc;
+            std::vector<Inst *> next;
+            vd.clear();
+            for (size_t i = 0; i < act.size(); ++i) {
+                Inst &I = *act[i];
+                if (std::sqrt(dv[i]) < m_o.tol) continue;
+                VecDesc d;
+                d.n = I.n;
+                d.add(I.v[V_SPQ], 1.0, I.v[V_SPV]);
+                vd.push_back(d);
+                next.push_back(&I);
+            }
+            if ((rc = m_dev.vecops(vd))) return rc;
+            act.swap(next);
+        }
+        return KSH_SUCCESS;
+    }
+
+    // _testSingularPoint(jacobian = false) (gsALMBase.hpp:475-511): power iterations with the
+    // factor of the step's counted tangent; |V . F| < SingularPointTestTol: a bifurcation point.
+    // Then the solution goes back to the driver's previous point (gsALMBase.hpp:427-428).
+    int sp_test(std::vector<Inst *> &g)
+    {
+        int rc;
+        if ((rc = sp_power(g, m_sp_test_it, false))) return rc;
+        keep_live(g);
+        std::vector<DotDesc> dd;
+        for (Inst *I : g) dd.push_back(dot1(*I, I->v[V_SPV], I->pb->F));
+        std::vector<double> dv;
+        if ((rc = m_dev.dots(dd, dv))) return rc;
+        std::vector<VecDesc> vd;
+        for (size_t i = 0; i < g.size(); ++i) {
+            Inst &I = *g[i];
+            I.sp_test = std::abs(dv[i]) < m_sp_test_tol;
+            VecDesc d;
+            d.n = I.n;
+            d.add(I.v[V_U], 1.0, I.v[V_SPUOLD]);
+            vd.push_back(d);
+            I.L = I.sp_Lold;
+            I.sp_conv = false;
+            I.sp = !I.sp_test ? Sp::Finish : m_o.sp_tol_b != 0 ? Sp::BisectStart : Sp::ExtStart;
+            if (m_o.verbose)
+                std::printf("[ksh path] singular point test: |V.F| = %.3e -> %s\n", std::abs(dv[i]),
+                            I.sp_test ? "bifurcation point" : "limit point");
+        }
+        return m_dev.vecops(vd);
+    }
+
+    // _bisectionSolve (gsALMBase.hpp:754-840): the reference value of the termination function
+    // and the pivot count at the start point, then the first step
+    int sp_bisect_start(std::vector<Inst *> &g)
+    {
+        int rc;
+        if ((rc = sp_tangent(g, V_U))) return rc;
+        keep_live(g);
+        std::vector<VecDesc> vd;
+        for (Inst *I : g) {
+            count_pivots(*I);
+            I->stability = I->indicator < 0 ? 1 : -1;
+            I->sp_ref = I->indicator;
+            I->sp_fa = I->negatives;
+            I->sp_Lbu = I->L;
+            I->sp_arc = I->arc;
+            I->sp_k = 1;
+            VecDesc d;
+            d.n = I->n;
+            d.add(I->v[V_SPBU], 1.0, I->v[V_U]);
+            vd.push_back(d);
+            I->sp = Sp::BisectStep;
+            I->ph = Phase::Start;
+        }
+        return m_dev.vecops(vd);
+    }
+
+    // after a bisection step: the pivot count at the new point (or at the start point when the
+    // step failed), halve the arc length when it changed, stop when |min D / min D_0| < TolB
+    int sp_bisect_eval(std::vector<Inst *> &g)
+    {
+        int rc;
+        if ((rc = sp_tangent(g, V_U))) return rc;
+        keep_live(g);
+        std::vector<VecDesc> vd;
+        for (Inst *I : g) {
+            count_pivots(*I);
+            I->stability = I->indicator < 0 ? 1 : -1;
+            VecDesc d;
+            d.n = I->n;
+            if (I->sp_fa == I->negatives && I->sp_step_ok) {
+                d.add(I->v[V_SPBU], 1.0, I->v[V_U]);
+                I->sp_Lbu = I->L;
+            } else {
+                I->arc = I->arc / 2.0;
+            }
+            const double term = I->indicator / I->sp_ref;
+            if (std::abs(term) < m_o.sp_tol_b || I->sp_k >= m_o.max_iter - 1) {
+                I->sp = Sp::BisectVector;
+            } else {
+                ++I->sp_k;
+                d.add(I->v[V_U], 1.0, I->v[V_SPBU]);
+                I->L = I->sp_Lbu;
+                I->sp = Sp::BisectStep;
+                I->ph = Phase::Start;
+            }
+            vd.push_back(d);
+        }
+        return m_dev.vecops(vd);
+    }
+
+    // the eigenvector at the bisection's last point, the arc length restored
+    int sp_bisect_vector(std::vector<Inst *> &g)
+    {
+        int rc;
+        if ((rc = sp_power(g, 5, true))) return rc;
+        for (Inst *I : g) {
+            if (I->ph == Phase::Done) continue;
+            I->arc = I->sp_arc;
+            I->sp = Sp::ExtStart;
+        }
+        return KSH_SUCCESS;
+    }
+
+    // _extendedSystemSolve (gsALMBase.hpp:631-650): K and the residual at the start point, zero
+    // increments
+    int sp_ext_start(std::vector<Inst *> &g)
+    {
+        int rc;
+        for (Inst *I : g)
+            if (!I->Keps) {
+                ksh_matrix *K = nullptr;
+                if ((rc = ksh_matrix_create(static_cast<ksh_problem *>(I->pb), KSH_FP64, &K))) return rc;
+                m_mats.ms.push_back(K);
+                I->Keps = K;
+            }
+        if ((rc = sp_tangent(g, V_U))) return rc;
+        keep_live(g);
+        std::vector<ResidualJob> rj;
+        std::vector<VecDesc> vd;
+        for (Inst *I : g) {
+            rj.push_back(ResidualJob{I->pb, I->v[V_U], I->L, KSH_RES_ALM, I->v[V_R]});
+            ++I->resid;
+            VecDesc d;
+            d.n = I->n;
+            d.add(I->v[V_DU], 0.0, I->v[V_DU]);
+            d.add(I->v[V_SPDV], 0.0, I->v[V_SPDV]);
+            d.add(I->v[V_X], 1.0, I->v[V_U]);
+            d.add(I->v[V_SPW], 1.0, I->v[V_SPV]);
+            vd.push_back(d);
+            I->sp_DL = 0.0;
+            I->sp_k = 0;
+            I->sp = Sp::ExtIter;
+        }
+        if ((rc = m_dev.residual(rj))) return rc;
+        return m_dev.vecops(vd);
+    }
+
+    // one iteration of _extendedSystemSolve / _extendedSystemIteration (gsALMBase.hpp:651-735):
+    // Newton on R(U, L) = 0, K(U) W = 0, ||W|| = 1 with the derivative of K along W by a finite
+    // difference, K(X + eps W) - K(X) over eps (eps = 1e-8):
+    //   K dUt = F, K dUb = -R,  h1 = (K_eps dUt - F) / eps,  h2 = K W + (K_eps dUb + R) / eps,
+    //   K dVt = -h1, K dVb = -h2,  dL = -(e . dVb + |W| - 1) / (e . dVt) with e = W / |W|,
+    //   DU += dL dUt + dUb, DV += dL dVt + dVb, DL += dL;
+    // converged when ||K(U + DU) (V + DV)|| < SingularPointComputeTolE
+    int sp_ext_iter(std::vector<Inst *> &g)
+    {
+        int rc;
+        const double eps = 1e-8, ie = 1.0 / eps;
+        std::vector<VecDesc> vd;
+        std::vector<SolveJob> sj;
+        for (Inst *I : g) {
+            VecDesc d;
+            d.n = I->n;
+            d.add(I->v[V_NR], -1.0, I->v[V_R]);
+            d.add(I->v[V_SPY], 1.0, I->v[V_X], eps, I->v[V_SPW]);
+            vd.push_back(d);
+            sj.push_back(SolveJob{I->K, I->pb->F, I->v[V_UT], m_o.max_refine, m_o.refine_tol, KSH_SUCCESS, I});
+            sj.push_back(SolveJob{I->K, I->v[V_NR], I->v[V_UB], m_o.max_refine, m_o.refine_tol, KSH_SUCCESS, I});
+        }
+        if ((rc = m_dev.vecops(vd)) || (rc = sp_solve(sj))) return rc;
+        keep_live(g);
+        std::vector<TangentJob> tj;
+        for (Inst *I : g) {
+            tj.push_back(TangentJob{I->pb, I->Keps, I->v[V_SPY]});
+            ++I->jac;
+        }
+        if ((rc = m_dev.assemble(tj))) return rc;
+        std::vector<ProductJob> pj;
+        for (Inst *I : g) {  // -K_eps dUt, -K_eps dUb, -K W
+            pj.push_back(ProductJob{I->Keps, I->v[V_UT], nullptr, I->v[V_SPP]});
+            pj.push_back(ProductJob{I->Keps, I->v[V_UB], nullptr, I->v[V_SPQ]});
+            pj.push_back(ProductJob{I->K, I->v[V_SPW], nullptr, I->v[V_T1]});
+        }
+        std::vector<double> rr;
+        if ((rc = m_dev.products(pj, rr))) return rc;
+        vd.clear();
+        sj.clear();
+        for (Inst *I : g) {  // -h1 and -h2
+            VecDesc d;
+            d.n = I->n;
+            d.add(I->v[V_T2], ie, I->pb->F, ie, I->v[V_SPP]);
+            d.add(I->v[V_SPY], 1.0, I->v[V_T1], ie, I->v[V_SPQ], -ie, I->v[V_R]);
+            vd.push_back(d);
+            sj.push_back(SolveJob{I->K, I->v[V_T2], I->v[V_SPVT], m_o.max_refine, m_o.refine_tol, KSH_SUCCESS, I});
+            sj.push_back(SolveJob{I->K, I->v[V_SPY], I->v[V_SPVB], m_o.max_refine, m_o.refine_tol, KSH_SUCCESS, I});
+        }
+        if ((rc = m_dev.vecops(vd)) || (rc = sp_solve(sj))) return rc;
+        keep_live(g);
+        std::vector<DotDesc> dd;
+        for (Inst *I : g) {
+            DotDesc d = dot1(*I, I->v[V_SPW], I->v[V_SPW]);
+            add_pair(d, I->v[V_SPW], I->v[V_SPVB]);
+            add_pair(d, I->v[V_SPW], I->v[V_SPVT]);
+            dd.push_back(d);
+        }
+        std::vector<double> dv;
+        if ((rc = m_dev.dots(dd, dv))) return rc;
+        vd.clear();
+        for (size_t i = 0; i < g.size(); ++i) {
+            Inst &I = *g[i];
+            const double nw = std::sqrt(dv[3 * i]);
+            const double dl = -(dv[3 * i + 1] / nw + nw - 1.0) / (dv[3 * i + 2] / nw);
+            I.sp_DL += dl;
+            VecDesc d;
+            d.n = I.n;
+            d.add(I.v[V_DU], 1.0, I.v[V_DU], dl, I.v[V_UT], 1.0, I.v[V_UB]);
+            d.add(I.v[V_SPDV], 1.0, I.v[V_SPDV], dl, I.v[V_SPVT], 1.0, I.v[V_SPVB]);
+            d.add(I.v[V_X], 1.0, I.v[V_U], 1.0, I.v[V_DU]);
+            d.add(I.v[V_SPW], 1.0, I.v[V_SPV], 1.0, I.v[V_SPDV]);
+            vd.push_back(d);
+        }
+        if ((rc = m_dev.vecops(vd))) return rc;
+        // K and the residual at the new point (the next iteration's), ||K (V + DV)||
+        if ((rc = sp_tangent(g, V_X))) return rc;
+        keep_live(g);
+        std::vector<ResidualJob> rj;
+        pj.clear();
+        for (Inst *I : g) {
+            rj.push_back(ResidualJob{I->pb, I->v[V_X], I->L + I->sp_DL, KSH_RES_ALM, I->v[V_R]});
+            ++I->resid;
+            pj.push_back(ProductJob{I->K, I->v[V_SPW], nullptr, I->v[V_T1]});
+        }
+        if ((rc = m_dev.residual(rj)) || (rc = m_dev.products(pj, rr))) return rc;
+        vd.clear();
+        for (size_t i = 0; i < g.size(); ++i) {
+            Inst &I = *g[i];
+            const double res = std::sqrt(rr[i]);
+            if (m_o.verbose)
+                std::printf("[ksh path] extended iteration %d: ||K phi|| = %.3e, lambda %.10e\n", I.sp_k, res,
+                            I.L + I.sp_DL);
+            const bool conv = res < m_sp_tol_e;
+            const bool last = I.sp_k == m_o.max_iter - 1;
+            if (!conv && !last) {
+                ++I.sp_k;
+                continue;
+            }
+            I.sp_conv = conv;
+            if (conv || m_o.sp_fail == 1) {  // gsALMBase.hpp:670-686
+                VecDesc d;
+                d.n = I.n;
+                d.add(I.v[V_U], 1.0, I.v[V_X]);
+                vd.push_back(d);
+                I.L += I.sp_DL;
+            }
+            I.sp = Sp::Finish;
+        }
+        return m_dev.vecops(vd);
+    }
+
+    // the end of computeSingularPoint (m_stability = m_stabilityPrev after a bifurcation point,
+    // gsALMBase.hpp:441-444), switchBranch with singular_point = 2 (gsALMBase.hpp:843-856:
+    // U += (||V|| / tau) V with V normalised, DUold = 0, DLold = 0), the drivers' new arc length,
+    // then the step's record
+    int sp_finish(std::vector<Inst *> &g)
+    {
+        int rc;
+        std::vector<DotDesc> dd;
+        std::vector<double> dv;
+        if (m_o.singular_point == 2) {
+            for (Inst *I : g) dd.push_back(dot1(*I, I->v[V_SPV], I->v[V_SPV]));
+            if ((rc = m_dev.dots(dd, dv))) return rc;
+            std::vector<VecDesc> vd;
+            for (size_t i = 0; i < g.size(); ++i) {
+                Inst &I = *g[i];
+                const double xi = 1.0 / m_sp_tau;  // ||V|| = 1 after the normalisation
+                VecDesc d;
+                d.n = I.n;
+                d.add(I.v[V_SPV], 1.0 / std::sqrt(dv[i]), I.v[V_SPV]);
+                d.add(I.v[V_U], 1.0, I.v[V_U], xi, I.v[V_SPV]);
+                d.add(I.v[V_DUOLD], 0.0, I.v[V_DUOLD]);
+                vd.push_back(d);
+                I.DLold = 0.0;
+                I.prev_zero = true;
+            }
+            if ((rc = m_dev.vecops(vd))) return rc;
+        }
+        std::vector<VecDesc> vd;
+        dd.clear();
+        for (Inst *I : g) {
+            DotDesc d = dot1(*I, I->v[V_U], I->v[V_U]);
+            if (m_o.method == 1) {
+                VecDesc e;
+                e.n = I->n;
+                e.add(I->v[V_T1], 1.0, I->v[V_U], -1.0, I->v[V_UPREV]);
+                vd.push_back(e);
+                add_pair(d, I->v[V_T1], I->v[V_T1]);
+            }
+            dd.push_back(d);
+        }
+        if ((rc = m_dev.vecops(vd)) || (rc = m_dev.dots(dd, dv))) return rc;
+        size_t o = 0;
+        for (Inst *I : g) {
+            I->unorm_new = std::sqrt(dv[o++]);
+            if (m_o.method == 1) I->prev_dnorm = std::sqrt(dv[o++]);
+            if (I->sp_test) I->stability = I->stability_prev;
+            I->sp_kind = !I->sp_test ? 1 : I->sp_conv ? 2 : 3;
+            if (m_o.length_after_sp > 0) I->arc0 = m_o.length_after_sp;
+            I->arc = I->arc_prev = I->arc0;
+            I->sp = Sp::None;
+        }
+        return record_steps(g);
     }
 
     ksh_alm_options m_o;
@@ -891,7 +1414,10 @@ private:
     Slab m_slab;
     MatrixOwner m_mats;
     double *m_coef = nullptr;
+    double *m_ones = nullptr;  // device ones (singular points)
     DevScratch m_decdesc, m_featdesc, m_feat;
+    double m_sp_test_tol = 1e-6, m_sp_tol_e = 1e-10, m_sp_tau = 1e3;
+    int m_sp_test_it = 5;
 };
 
 }  // namespace

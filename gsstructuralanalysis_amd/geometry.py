@@ -227,6 +227,53 @@ def simply_supported_plate_problem(p=3, n=8, q=1.0, L=1.0, t=0.01, E=1e7, nu=0.3
     return pt
 
 
+def rectangle(p: int, nx: int, ny: int, a: float, b: float, t: float, E: float, nu: float, rho: float = 1.0) -> Patch:
+    """Flat rectangle [0, a] x [0, b] with nx x ny elements."""
+
+    def fun(xi, eta):
+        return np.stack([a * xi, b * eta, np.zeros_like(xi)], axis=-1)
+
+    return Patch(p, nx, ny, interpolate_surface(p, nx, ny, fun), t, E, nu, rho)
+
+
+def add_edge_load(pt: Patch, side: str, q) -> Patch:
+    """Uniform dead line load q (force per unit undeformed length) on one straight edge, as point
+    loads at the edge's Gauss points ((p + 1) per element, weights times the edge length): the
+    consistent nodal forces of the line load, exactly."""
+    gx, gw = np.polynomial.legendre.leggauss(pt.p + 1)
+    ne = pt.ny if side in ("west", "east") else pt.nx
+    x0 = pt.ctrl.reshape(pt.my, pt.mx, 3)
+    length = np.linalg.norm(x0[-1, 0] - x0[0, 0]) if side in ("west", "east") else np.linalg.norm(x0[0, -1] - x0[0, 0])
+    q = np.asarray(q, dtype=np.float64)
+    for e in range(ne):
+        for s, w in zip(gx, gw):
+            t = (e + 0.5 * (s + 1.0)) / ne
+            wt = 0.5 * w / ne * length
+            xi, eta = {"west": (0.0, t), "east": (1.0, t), "south": (t, 0.0), "north": (t, 1.0)}[side]
+            pt.add_point_load(xi, eta, wt * q)
+    return pt
+
+
+# the clamped strip of examples/gsThinShell_ArcLength.cpp:186-195,455-487 (test case 1): EI = 1
+STRIP_T = 0.005313292845913 * 2
+
+
+def compressed_strip_problem(p=3, nx=16, ny=1, a=1.0, b=0.1, t=STRIP_T, E=1e8, nu=0.0) -> Patch:
+    """Strip a x b, both ends clamped, compressed by the dead line load E b t 1e-6 on the west
+    edge (x free there), y held on every edge; perfect and flat, so the compressed state loses
+    stability at a symmetric bifurcation, the clamped Euler load 4 pi^2 E I / a^2 (total force;
+    E I = E b t^3 / 12 = 1 for the default sizes)."""
+    pt = rectangle(p, nx, ny, a, b, t, E, nu)
+    pt.fix("west", (1, 2))
+    pt.clamp("west", (2,))
+    pt.fix("east", (0, 1, 2))
+    pt.clamp("east", (2,))
+    pt.fix("south", (1,))
+    pt.fix("north", (1,))
+    add_edge_load(pt, "west", (E * b * t * 1e-6, 0.0, 0.0))
+    return pt
+
+
 def roof_problem(p=3, n=32, P=-10.0, **kw) -> Patch:
     """Hinged straight edges (south/north), free curved edges, central point load P in z."""
     pt = shallow_roof(p, n, **kw)

@@ -29,14 +29,19 @@ def _f64(a):
 def alm_options(nsteps, length, method=0, precision=L.FP64, max_iter=20, tol=1e-6, tol_u=1e-6, tol_f=1e-3,
                 scaling=-1.0, angle_method=0, bifurcation_method=0, relaxation=1.0, quasi=0, quasi_iterations=-1,
                 adaptive=0, adaptive_iterations=10, max_refine=0, refine_tol=1e-14, max_halvings=10, verbose=0,
-                inertia_guard=0.0, learned=None, refine_accept=0.0, learned_mode=0):
+                inertia_guard=0.0, learned=None, refine_accept=0.0, learned_mode=0, singular_point=0,
+                sp_test_tol=1e-6, sp_test_it=5, sp_tol_e=1e-10, sp_tol_b=0.0, perturbation=1e3, sp_fail=1,
+                length_after_sp=0.0):
     """ksh_alm_options with the reference's option names in snake case (gsALMBase.hpp:22-54);
     `learned` is a LearnedPredictor (Crisfield only); learned_mode 1 keeps the predictor's
-    extrapolation and leaves out the network's correction (ablation)."""
+    extrapolation and leaves out the network's correction (ablation); singular_point 1 / 2:
+    computeSingularPoint on a change of stability, 2 followed by switchBranch."""
     return L.AlmOptions(method, precision, nsteps, length, max_iter, tol, tol_u, tol_f, scaling, angle_method,
                         bifurcation_method, relaxation, quasi, quasi_iterations, adaptive, adaptive_iterations,
                         max_refine, refine_tol, max_halvings, verbose, inertia_guard,
-                        learned.handle if learned is not None else None, refine_accept, learned_mode)
+                        learned.handle if learned is not None else None, refine_accept, learned_mode, None,
+                        singular_point, sp_test_tol, sp_test_it, sp_tol_e, sp_tol_b, perturbation, sp_fail,
+                        length_after_sp)
 
 
 def _alm_result(nsteps, nfree, keep_U, probe_dof, features=False):
@@ -45,6 +50,7 @@ def _alm_result(nsteps, nfree, keep_U, probe_dof, features=False):
         out["features"] = np.zeros((nsteps, 66))
     out["negatives"] = np.zeros(nsteps, dtype=np.int64)
     out["iterations"] = np.zeros(nsteps, dtype=np.int32)
+    out["singular"] = np.zeros(nsteps, dtype=np.int32)
     U = np.zeros(nsteps * nfree) if keep_U else None
     r = L.AlmResult()
     r.lambda_ = _dptr(out["lambda"])
@@ -58,6 +64,7 @@ def _alm_result(nsteps, nfree, keep_U, probe_dof, features=False):
     r.probe_dof = probe_dof
     r.U = _dptr(U) if U is not None else None
     r.features = _dptr(out["features"]) if features else None
+    r.singular = out["singular"].ctypes.data_as(C.POINTER(C.c_int))
     return r, out, U
 
 

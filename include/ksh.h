@@ -207,6 +207,22 @@ typedef struct {
                              /* quadratic extrapolation alone (network output taken as 0)  */
     const double *lengths;   /* ksh_batch_run_alm: arc length of each problem (NULL: all    */
                              /* take `length`)                                              */
+    /* singular points: on a change of stability after a converged step the driver calls
+     * computeSingularPoint from the previous converged point (gsALMBase.hpp:422-446): the
+     * singular-point test, optionally a bisection on the pivot count, the extended system for
+     * the critical point; then, with singular_point = 2, switchBranch (gsALMBase.hpp:843-856).
+     * The arc length restarts from length_after_sp (the drivers' "Length2" / dL).            */
+    int singular_point;      /* 0 off; 1 solvers/quasistatic_shell_XML.cpp:305-311; 2 with  */
+                             /* the branch switch, examples/gsThinShell_ArcLength.cpp:1107-1117 */
+    double sp_test_tol;      /* "SingularPointTestTol" (0: 1e-6)                          */
+    int sp_test_it;          /* "SingularPointTestIt" (0: 5)                              */
+    double sp_tol_e;         /* "SingularPointComputeTolE" (0: 1e-10)                     */
+    double sp_tol_b;         /* "SingularPointComputeTolB" (0: no bisection)              */
+    double perturbation;     /* "Perturbation" tau of switchBranch (0: 1e3)               */
+    int sp_fail;             /* "SingularPointFailure": 1 (the reference's default) keeps  */
+                             /* the last extended iterate when they do not converge, 0     */
+                             /* returns to the start point                                 */
+    double length_after_sp;  /* arc length after a singular point (0: `length`)           */
 } ksh_alm_options;
 
 typedef struct {
@@ -229,6 +245,9 @@ typedef struct {
     double *features;        /* [nsteps*66] predictor features of each converged state (opt.) */
     int64_t learned_steps;   /* steps that started from the learned proposal              */
     int64_t fallbacks;       /* of those, steps redone from the tangent predictor         */
+    int *singular;           /* [nsteps] 0; singular point handled after the step: 1 limit */
+                             /* point (solution set back), 2 bifurcation point computed,   */
+                             /* 3 bifurcation, extended iterations not converged (opt.)    */
 } ksh_alm_result;
 
 /* benchmarks/benchmark_Roof.cpp:320-512: options, initialize, step loop with halving. */

@@ -60,7 +60,7 @@ class ALMCrisfield:
     """gsALMBase::_step (gsALMBase.hpp:353-416) with gsALMCrisfield's methods (gsALMCrisfield.hpp:40-472)."""
 
     def __init__(self, ops: OracleOps, length, max_iter=100, tol_f=1e-3, tol_u=1e-6, scaling=-1.0, angle_method=0,
-                 relaxation=1.0, bifurcation_method=0, faithful=False):
+                 relaxation=1.0, bifurcation_method=0, faithful=False, tol=1e-6):
         self.ops = ops
         self.F = ops.F
         n = ops.n
@@ -88,6 +88,8 @@ class ALMCrisfield:
         self.jac = None
         self.fac = None
         self.num_iter = 0
+        self.tol = tol
+        self.V = np.zeros(n)
 
     # gsALMBase::init -> _computeStability(m_U) (gsALMBase.hpp:95-103)
     def initialize(self):
@@ -271,6 +273,126 @@ class ALMCrisfield:
     def set_solution(self, U, L):
         self.U, self.L = U.copy(), L
 
+    # ---- singular points: gsALMBase.hpp:422-446 (_computeSingularPoint), 475-511
+    # (_testSingularPoint), 631-735 (extended system), 754-840 (bisection), 843-856 (switchBranch)
+    def _power(self, V, its, stop_tol=None):
+        """V <- K^-1 V / ||K^-1 V|| with the factor of the current tangent; stop_tol: the
+        bisection's early exit ||V - Vold|| < Tol (gsALMBase.hpp:824-837)."""
+        Vold = np.zeros_like(V)
+        for _ in range(its):
+            V = self.ops.solve(self.fac, V)
+            V = V / np.linalg.norm(V)
+            if stop_tol is not None:
+                if np.linalg.norm(V - Vold) < stop_tol:
+                    return V, True
+                Vold = V
+        return V, False
+
+    def test_singular_point(self, test_tol=1e-6, test_it=5):
+        """_testSingularPoint(jacobian=false): power iterations from the normalised ones vector
+        with the factor of the last counted tangent; a bifurcation point when |V . F| < tol."""
+        n = self.ops.n
+        self.V, _ = self._power(np.ones(n) / np.sqrt(n), test_it)
+        return abs(self.V @ self.F) < test_tol
+
+    def extended_system_solve(self, U, L, tol, sp_fail=1):
+        """_extendedSystemSolve / _extendedSystemIteration: Newton on R(U, L) = 0, K(U) W = 0,
+        ||W|| = 1 from (U, L, V), the derivative of K along W by a finite difference (eps 1e-8);
+        stops when ||K(U + DU) (V + DV)|| < tol.  Returns whether it converged."""
+        self.U, self.L = U.copy(), L
+        n = self.ops.n
+        self._compute_jacobian(self.U)
+        DU, DV, DL = np.zeros(n), np.zeros(n), 0.0
+        F = self.F
+        eps = 1e-8
+        for it in range(self.max_iter):
+            X = self.U + DU
+            R = self.ops.residual_alm(X, self.L + DL)
+            self._compute_jacobian(X)
+            Ut = self.ops.solve(self.fac, F)
+            Ub = self.ops.solve(self.fac, -R)
+            W = self.V + DV
+            Keps = self.ops.jacobian(X + eps * W)
+            h1 = (band_symv(Keps, n, self.ops.bw, Ut) - F) / eps
+            h2 = band_symv(self.jac, n, self.ops.bw, W) + (band_symv(Keps, n, self.ops.bw, Ub) + R) / eps
+            Vt = self.ops.solve(self.fac, -h1)
+            Vb = self.ops.solve(self.fac, -h2)
+            nw = np.linalg.norm(W)
+            e = W / nw
+            dl = -(e @ Vb + nw - 1.0) / (e @ Vt)
+            DU = DU + dl * Ut + Ub
+            DV = DV + dl * Vt + Vb
+            DL += dl
+            self._compute_jacobian(self.U + DU)
+            self.sp_residual = np.linalg.norm(band_symv(self.jac, n, self.ops.bw, self.V + DV))
+            self.sp_iterations = it + 1
+            if self.sp_residual < tol:
+                self.U = self.U + DU
+                self.L += DL
+                return True
+            if it == self.max_iter - 1:
+                if sp_fail == 1:
+                    self.U = self.U + DU
+                    self.L += DL
+                return False
+        return False
+
+    def bisection_solve(self, U, L, tol):
+        """_bisectionSolve: arc-length steps from the last point whose pivot count did not change,
+        halving the arc length when the count changes or the step fails, until
+        |min D / min D(start)| < tol; then the eigenvector V by power iterations."""
+        self.U, self.L = U.copy(), L
+        U_old, L_old = U.copy(), L
+        dL = self.arc
+        self._compute_jacobian(self.U)
+        self._stability()
+        ref = self.indicator
+        fa = self.negatives
+        converged = False
+        for _ in range(1, self.max_iter):
+            self.U, self.L = U_old.copy(), L_old
+            st = self.step()
+            self._compute_jacobian(self.U)
+            self._stability()
+            fb = self.negatives
+            if fa == fb and st == 0:
+                U_old, L_old = self.U.copy(), self.L
+                fa = fb
+            else:
+                self.arc = self.arc / 2.0
+            if abs(self.indicator / ref) < tol:
+                converged = True
+                break
+        self.arc = dL
+        n = self.ops.n
+        self.V, hit = self._power(np.ones(n) / np.sqrt(n), 5, stop_tol=self.tol)
+        return converged or hit
+
+    def switch_branch(self, tau):
+        """switchBranch: U += (||V|| / tau) V with V normalised; the previous increment forgotten."""
+        self.V = self.V / np.linalg.norm(self.V)
+        xi = np.linalg.norm(self.V) / tau
+        self.U = self.U + xi * self.V
+        self.DeltaLold = 0.0
+        self.DeltaUold = np.zeros_like(self.DeltaUold)
+
+    def compute_singular_point(self, U, L, switch=False, test_tol=1e-6, test_it=5, tol_b=0.0, tol_e=1e-10,
+                               sp_fail=1, tau=1e3):
+        """_computeSingularPoint(U, L, switchBranch, jacobian=false, testPoint=true).  Returns
+        0 limit point (the solution is set back to (U, L)), 2 bifurcation point computed,
+        3 bifurcation point whose extended iterations did not converge."""
+        test = self.test_singular_point(test_tol, test_it)
+        self.U, self.L = U.copy(), L
+        if not test:
+            return 0
+        if tol_b != 0:
+            self.bisection_solve(self.U, self.L, tol_b)
+        conv = self.extended_system_solve(self.U, self.L, tol_e, sp_fail)
+        if switch and (conv or sp_fail == 1):
+            self.switch_branch(tau)
+        self.stability = self.stability_prev
+        return 2 if conv else 3
+
     def set_length(self, length):
         """setLength (gsALMBase.h:118-122): the arc length and its 'previous' value."""
         self.arc = length
@@ -327,11 +449,18 @@ class ALMRiks(ALMCrisfield):
         self.arc_prev = self.arc
 
 
-def run_crisfield(patch, nsteps, length, max_halvings=10, method=0, ops=None, keep_jac=False, **kw):
+def run_crisfield(patch, nsteps, length, max_halvings=10, method=0, ops=None, keep_jac=False, singular_point=0,
+                  sp_test_tol=1e-6, sp_test_it=5, sp_tol_b=0.0, sp_tol_e=1e-10, sp_fail=1, perturbation=1e3,
+                  length_after_sp=0.0, **kw):
     """Driver loop of benchmarks/benchmark_Roof.cpp:417-512 on the oracle (method 0 Crisfield, 1 Riks).
 
     ``ops``: any object with the OracleOps interface (default: the shell oracle of ``patch``);
-    ``keep_jac``: also return the counted tangent (band) of every converged step."""
+    ``keep_jac``: also return the counted tangent (band) of every converged step;
+    ``singular_point``: on a change of stability after a converged step, 1 computeSingularPoint
+    from the previous point (solvers/quasistatic_shell_XML.cpp:305-311), 2 the same followed by
+    switchBranch (examples/gsThinShell_ArcLength.cpp:1107-1117); the arc length then restarts
+    from ``length_after_sp`` (0: the initial length).  ``singular`` records per step 0, or 1 a
+    limit point, 2 a computed bifurcation point, 3 one whose extended iterations did not converge."""
     ops = ops if ops is not None else OracleOps(patch)
     alm = (ALMCrisfield if method == 0 else ALMRiks)(ops, length, **kw)
     t0 = time.perf_counter()
@@ -340,7 +469,7 @@ def run_crisfield(patch, nsteps, length, max_halvings=10, method=0, ops=None, ke
     dL0 = dL = length
     bisected = False
     out = {"lambda": [], "unorm": [], "negatives": [], "indicator": [], "iterations": [], "length": [], "U": [],
-           "residue_f": [], "phi_step": []}
+           "residue_f": [], "phi_step": [], "singular": []}
     if keep_jac:
         out["jac"] = []
     halvings = 0
@@ -358,16 +487,29 @@ def run_crisfield(patch, nsteps, length, max_halvings=10, method=0, ops=None, ke
             continue
         if st != 0:
             break
+        used = dL
+        sing = 0
+        if singular_point and alm.stability * alm.stability_prev < 0:
+            kind = alm.compute_singular_point(Uold, Lold, False, sp_test_tol, sp_test_it, sp_tol_b, sp_tol_e, sp_fail,
+                                              perturbation)
+            sing = 1 if kind == 0 else kind
+            if singular_point == 2:
+                alm.switch_branch(perturbation)
+            if length_after_sp > 0:
+                dL0 = length_after_sp
+            dL = dL0
+            alm.set_length(dL)
         Uold, Lold = alm.U.copy(), alm.L
         out["lambda"].append(Lold)
         out["unorm"].append(np.linalg.norm(Uold))
         out["negatives"].append(alm.negatives)
         out["indicator"].append(alm.indicator)
         out["iterations"].append(alm.num_iter)
-        out["length"].append(dL)
+        out["length"].append(used)
         out["U"].append(Uold)
         out["residue_f"].append(alm.residueF)
         out["phi_step"].append(alm.phi)
+        out["singular"].append(sing)
         if keep_jac:
             out["jac"].append(np.array(alm.jac, copy=True))
         if not bisected:
