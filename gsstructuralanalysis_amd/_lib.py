@@ -69,6 +69,30 @@ class StaticResult(C.Structure):
                 ("wall_seconds", C.c_double), ("jacobians", C.c_int64)]
 
 
+class DrOptions(C.Structure):
+    _fields_ = [("max_iter", C.c_int), ("tol", C.c_double), ("tol_f", C.c_double), ("tol_u", C.c_double),
+                ("damping", C.c_double), ("alpha", C.c_double), ("tol_e", C.c_double), ("reset_it", C.c_int),
+                ("verbose", C.c_int)]
+
+
+class DrResult(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("final_status", C.c_int), ("peaks", C.c_int), ("residual", C.c_double),
+                ("ek", C.c_double), ("ek0", C.c_double), ("U", _dp), ("ek_history", _dp),
+                ("wall_seconds", C.c_double), ("residuals", C.c_int64)]
+
+
+class NewmarkOptions(C.Structure):
+    _fields_ = [("precision", C.c_int), ("nsteps", C.c_int), ("dt", C.c_double), ("beta", C.c_double),
+                ("gamma", C.c_double), ("damping_mass", C.c_double), ("max_iter", C.c_int), ("tol_f", C.c_double),
+                ("tol_u", C.c_double), ("max_refine", C.c_int), ("refine_tol", C.c_double), ("verbose", C.c_int)]
+
+
+class NewmarkResult(C.Structure):
+    _fields_ = [("steps_done", C.c_int), ("final_status", C.c_int), ("time", _dp), ("unorm", _dp),
+                ("iterations", _ip), ("U", _dp), ("V", _dp), ("A", _dp), ("w", _dp), ("probe_dof", C.c_int),
+                ("wall_seconds", C.c_double), ("jacobians", C.c_int64)]
+
+
 EIG_SHIFT_INVERT, EIG_BUCKLING = 0, 1
 
 
@@ -124,6 +148,11 @@ SIGNATURES = {
     "ksh_run_alm": (C.c_int, [_VP, C.POINTER(AlmOptions), C.POINTER(AlmResult)]),
     "ksh_batch_run_alm": (C.c_int, [C.POINTER(_VP), C.c_int, C.POINTER(AlmOptions), C.POINTER(AlmResult), _dp]),
     "ksh_run_static": (C.c_int, [_VP, C.POINTER(StaticOptions), C.POINTER(StaticResult)]),
+    "ksh_run_dr": (C.c_int, [_VP, C.POINTER(DrOptions), C.POINTER(DrResult)]),
+    "ksh_batch_run_dr": (C.c_int, [C.POINTER(_VP), C.c_int, C.POINTER(DrOptions), C.POINTER(DrResult), _dp]),
+    "ksh_run_newmark": (C.c_int, [_VP, C.POINTER(NewmarkOptions), _dp, _dp, _dp, C.POINTER(NewmarkResult)]),
+    "ksh_batch_run_newmark": (C.c_int, [C.POINTER(_VP), C.c_int, C.POINTER(NewmarkOptions), C.POINTER(NewmarkResult),
+                                        _dp]),
     "ksh_eigen": (C.c_int, [_VP, _VP, C.POINTER(EigOptions), _dp, _dp, C.POINTER(EigResult)]),
     "ksh_run_buckling": (C.c_int, [_VP, C.POINTER(EigOptions), _dp, _dp, C.POINTER(EigResult)]),
     "ksh_run_modal": (C.c_int, [_VP, C.POINTER(EigOptions), _dp, _dp, C.POINTER(EigResult)]),

@@ -12,7 +12,8 @@
 //   k_features  the 66 predictor features of a state: lambda, ||U||, and the 64 least-squares
 //               coefficients of the out-of-plane control displacements in the analytic modes
 //               (P [64][ncp] times the gathered z values);
-//   k_decode    dU[z dof of cp] += scale * sum_r Phi[cp][r] coef[r].
+//   k_decode    dU[z dof of cp] += scale * sum_r Phi[cp][r] coef[r];
+//   k_dr        one explicit dynamic-relaxation update of an instance with its four norms.
 // All are HBM / latency bound and tiny next to the assembly and the factor.
 #include <hip/hip_runtime.h>
 
@@ -140,6 +141,50 @@ __global__ __launch_bounds__(kT) void k_decode(const DecDesc *__restrict__ descs
     D.dU[f] += D.scale * s;
 }
 
+__global__ __launch_bounds__(kT) void k_dr(const DrDesc *__restrict__ descs)
+{
+    const DrDesc &D = descs[blockIdx.x];
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    __shared__ double red[4][kT / 64];
+    const double dt = D.dt;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < D.n; i += kT) {
+        double r = D.R[i], v = D.v[i], du;
+        if (D.mode == 0) {
+            du = 0.5 * dt * dt * D.minv[i] * r;
+            v = du / dt;
+        } else if (D.mode == 1) {
+            r -= D.damp[i] * v;
+            v += dt * D.minv[i] * r;
+            du = dt * v;
+        } else {
+            r -= D.damp[i] * v;
+            du = -1.5 * dt * v + 0.5 * dt * dt * D.minv[i] * r;
+            v = 0.0;
+        }
+        const double DU = D.DU[i] + du;
+        D.v[i] = v;
+        D.dU[i] = du;
+        D.DU[i] = DU;
+        D.X[i] = D.U[i] + DU;
+        s[0] += r * r;
+        s[1] += v * (D.m[i] * v);
+        s[2] += du * du;
+        s[3] += DU * DU;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double w = wave_sum(s[q]);
+        if (lane == 0) red[q][wave] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double t = 0.0;
+        for (int w = 0; w < kT / 64; ++w) t += red[threadIdx.x][w];
+        D.out[threadIdx.x] = t;
+    }
+}
+
 template <typename Desc>
 int upload_descs(hipStream_t s, DevScratch &buf, const std::vector<Desc> &h)
 {
@@ -192,6 +237,18 @@ int batch_vecops(hipStream_t s, DevScratch &desc_buf, const std::vector<VecDesc>
     if ((rc = upload_descs(s, desc_buf, h))) return rc;
     hipLaunchKernelGGL(k_vecops, dim3((unsigned)((nmax + kT - 1) / kT), (unsigned)h.size()), dim3(kT), 0, s,
                        (const VecDesc *)desc_buf.p);
+    KSH_HIP(hipGetLastError());
+    return KSH_SUCCESS;
+}
+
+int batch_dr(hipStream_t s, DevScratch &desc_buf, const std::vector<DrDesc> &h)
+{
+    if (h.empty()) return KSH_SUCCESS;
+    for (const DrDesc &d : h)
+        if (d.n < 0 || d.mode < 0 || d.mode > 2 || !d.out) return fail(KSH_OTHER_ERROR, "dynamic relaxation descriptor");
+    int rc;
+    if ((rc = upload_descs(s, desc_buf, h))) return rc;
+    hipLaunchKernelGGL(k_dr, dim3((unsigned)h.size()), dim3(kT), 0, s, (const DrDesc *)desc_buf.p);
     KSH_HIP(hipGetLastError());
     return KSH_SUCCESS;
 }

@@ -79,4 +79,18 @@ int batch_vecops(hipStream_t s, DevScratch &desc_buf, const std::vector<VecDesc>
 int batch_features(hipStream_t s, DevScratch &desc_buf, const std::vector<FeatDesc> &h);
 int batch_decode(hipStream_t s, DevScratch &desc_buf, const std::vector<DecDesc> &h);
 
+// one explicit step of dynamic relaxation on one instance (gsStaticDR.hpp:182-266), dt given:
+//   mode 0 (_start)     dU = dt^2 / 2 minv R,              v = dU / dt
+//   mode 1 (_iteration) R -= damp v, v += dt minv R,       dU = dt v
+//   mode 2 (_peak)      R -= damp v, dU = -1.5 dt v + dt^2 / 2 minv R, v = 0
+// then DU += dU, X = U + DU; out = {||R||^2 (after the damping term), v . (m v), ||dU||^2, ||DU||^2}
+struct DrDesc {
+    int n = 0, mode = 0;
+    double dt = 1.0;
+    const double *R = nullptr, *m = nullptr, *minv = nullptr, *damp = nullptr, *U = nullptr;
+    double *v = nullptr, *dU = nullptr, *DU = nullptr, *X = nullptr;
+    double *out = nullptr;
+};
+int batch_dr(hipStream_t s, DevScratch &desc_buf, const std::vector<DrDesc> &h);
+
 }  // namespace ksh

@@ -298,3 +298,111 @@ class ShellProblem:
         if U is not None:
             res["U"] = U.reshape(nsteps, self.nfree)[:k]
         return res
+
+    def run_dr(self, keep_U=True, **kw):
+        """Dynamic relaxation (solvers/static_shell_XML.cpp:243-275 on the device)."""
+        return _dr_run([self], dr_options(**kw), keep_U)[0][0]
+
+    def run_newmark(self, nsteps, U0=None, V0=None, A0=None, keep=True, probe_dof=-1, **kw):
+        """Nonlinear Newmark from (U0, V0, A0) (tutorials/nonlinear_shell_dynamic.cpp:155-186 on the device)."""
+        o = newmark_options(nsteps, **kw)
+        r, out, arrs = _newmark_result(nsteps, self.nfree, keep, probe_dof)
+        ics = [None if v is None else _f64(v) for v in (U0, V0, A0)]
+        for v in ics:
+            if v is not None and v.shape != (self.nfree,):
+                raise ValueError("initial states have one entry per free dof")
+        code = L.load().ksh_run_newmark(self.handle, C.byref(o), *[(_dptr(v) if v is not None else None) for v in ics],
+                                        C.byref(r))
+        if code not in (L.SUCCESS, L.NOT_CONVERGED):
+            L.check(code)
+        return _newmark_collect(r, out, arrs)
+
+
+# ---- residual-only and mass consumers ---------------------------------------------------------
+def dr_options(max_iter=25, tol=1e-6, tol_f=-1.0, tol_u=-1.0, damping=1.0, alpha=2.0, tol_e=1e-6, reset_it=-1,
+               verbose=0):
+    """ksh_dr_options with gsStaticDR's option names in snake case (gsStaticDR.hpp:20-27,
+    gsStaticBase.h:66-72): maxIt, tol, tolF, tolU, damping, alpha, tolE, ResetIt."""
+    return L.DrOptions(max_iter, tol, tol_f, tol_u, damping, alpha, tol_e, reset_it, verbose)
+
+
+def _dr_run(problems, opts, keep_U):
+    n = len(problems)
+    res = (L.DrResult * n)()
+    Us, eks = [], []
+    for i, sp in enumerate(problems):
+        U = np.zeros(sp.nfree) if keep_U else None
+        ek = np.zeros(opts.max_iter)
+        res[i].U = _dptr(U) if U is not None else None
+        res[i].ek_history = _dptr(ek)
+        Us.append(U)
+        eks.append(ek)
+    handles = (C.c_void_p * n)(*[p.handle for p in problems])
+    wall = C.c_double()
+    code = L.load().ksh_batch_run_dr(handles, n, C.byref(opts), res, C.byref(wall))
+    if code not in (L.SUCCESS, L.NOT_CONVERGED):
+        L.check(code)
+    out = []
+    for i in range(n):
+        r = res[i]
+        d = dict(iterations=r.iterations, converged=r.final_status == L.SUCCESS, final_status=r.final_status,
+                 peaks=r.peaks, residual=r.residual, ek=r.ek, ek0=r.ek0, wall_seconds=r.wall_seconds,
+                 residuals=r.residuals, ek_history=eks[i][:r.iterations + 1])
+        if Us[i] is not None:
+            d["U"] = Us[i]
+        out.append(d)
+    return out, wall.value
+
+
+def run_dr_batch(problems, keep_U=False, **kw):
+    """Dynamic relaxation of many problems at once (ksh_batch_run_dr); (results, wall seconds)."""
+    return _dr_run(problems, dr_options(**kw), keep_U)
+
+
+def newmark_options(nsteps, dt=1e-2, precision=L.FP64, beta=0.25, gamma=0.5, damping_mass=0.0, max_iter=100,
+                    tol_f=1e-3, tol_u=1e-6, max_refine=0, refine_tol=1e-14, verbose=0):
+    """ksh_newmark_options (gsDynamicBase.hpp:20-35, gsDynamicNewmark.hpp:20-25: the reference's
+    "alpha" and "delta" are beta and gamma here)."""
+    return L.NewmarkOptions(precision, nsteps, dt, beta, gamma, damping_mass, max_iter, tol_f, tol_u, max_refine,
+                            refine_tol, verbose)
+
+
+def _newmark_result(nsteps, nfree, keep, probe_dof):
+    out = {k: np.zeros(nsteps) for k in ("time", "unorm", "w")}
+    out["iterations"] = np.zeros(nsteps, dtype=np.int32)
+    arrs = {k: np.zeros((nsteps, nfree)) for k in ("U", "V", "A")} if keep else {}
+    r = L.NewmarkResult()
+    r.time = _dptr(out["time"])
+    r.unorm = _dptr(out["unorm"])
+    r.w = _dptr(out["w"])
+    r.probe_dof = probe_dof
+    r.iterations = out["iterations"].ctypes.data_as(C.POINTER(C.c_int))
+    for k, a in arrs.items():
+        setattr(r, k, _dptr(a))
+    return r, out, arrs
+
+
+def _newmark_collect(r, out, arrs):
+    k = r.steps_done
+    res = {key: val[:k] for key, val in out.items()}
+    res.update({key: val[:k] for key, val in arrs.items()})
+    res.update(steps_done=k, final_status=r.final_status, wall_seconds=r.wall_seconds, jacobians=r.jacobians)
+    return res
+
+
+def run_newmark_batch(problems, nsteps, keep=False, probe_dof=-1, **kw):
+    """Newmark time integration of many problems from rest at once (ksh_batch_run_newmark)."""
+    o = newmark_options(nsteps, **kw)
+    n = len(problems)
+    res = (L.NewmarkResult * n)()
+    keepers = []
+    for i, sp in enumerate(problems):
+        r, out, arrs = _newmark_result(nsteps, sp.nfree, keep, probe_dof)
+        res[i] = r
+        keepers.append((out, arrs))
+    handles = (C.c_void_p * n)(*[p.handle for p in problems])
+    wall = C.c_double()
+    code = L.load().ksh_batch_run_newmark(handles, n, C.byref(o), res, C.byref(wall))
+    if code not in (L.SUCCESS, L.NOT_CONVERGED):
+        L.check(code)
+    return [_newmark_collect(res[i], *keepers[i]) for i in range(n)], wall.value

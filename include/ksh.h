@@ -293,6 +293,77 @@ typedef struct {
  * gsControlDisplacement::step (src/gsStaticSolvers/gsControlDisplacement.h:46-60). */
 int ksh_run_static(ksh_problem *pb, const ksh_static_options *opt, ksh_static_result *res);
 
+/* ---- residual-only and mass consumers -----------------------------------------
+ * Dynamic relaxation: gsStaticDR (src/gsStaticSolvers/gsStaticDR.hpp:93-266) over the static
+ * residual F - F_int(U) at the full load, with the lumped mass (assembleMass(true), the row
+ * sums of the consistent mass) as solvers/static_shell_XML.cpp:243-275 builds it; explicit
+ * steps dt = 1 from U = 0, viscous damping c m or kinetic damping (c = 0: velocities reset at
+ * each kinetic-energy peak).  Option names are gsStaticDR's and gsStaticBase's. */
+typedef struct {
+    int max_iter;            /* "maxIt"                                                   */
+    double tol;              /* "tol"                                                     */
+    double tol_f, tol_u;     /* "tolF", "tolU" (-1: tol)                                  */
+    double damping;          /* "damping" c                                               */
+    double alpha;            /* "alpha": fictitious mass alpha m                          */
+    double tol_e;            /* "tolE": kinetic energy over the first one                  */
+    int reset_it;            /* "ResetIt" (-1: never)                                     */
+    int verbose;
+} ksh_dr_options;
+
+typedef struct {
+    int iterations;
+    int final_status;        /* KSH_SUCCESS | KSH_NOT_CONVERGED (maximum iterations)       */
+    int peaks;               /* kinetic-damping resets                                    */
+    double residual;         /* ||R|| / ||R_0|| at the end                                 */
+    double ek, ek0;          /* kinetic energy at the end and after the first step         */
+    double *U;               /* [n] solution (optional)                                   */
+    double *ek_history;      /* [max_iter] kinetic energy per iteration (optional)        */
+    double wall_seconds;
+    int64_t residuals;       /* residual evaluations                                      */
+} ksh_dr_result;
+
+int ksh_run_dr(ksh_problem *pb, const ksh_dr_options *opt, ksh_dr_result *res);
+/* many problems at once, one residual launch per round for all; res[count] */
+int ksh_batch_run_dr(ksh_problem *const *pbs, int count, const ksh_dr_options *opt, ksh_dr_result *res,
+                     double *wall_seconds);
+
+/* Nonlinear Newmark: gsDynamicNewmark<T, true> (src/gsDynamicSolvers/gsDynamicNewmark.hpp:69-142)
+ * under the time loop of tutorials/nonlinear_shell_dynamic.cpp:176-186: per step the predictor
+ * with A = 0, then Newton on M A + C V = F - F_int(U) with the tangent M + gamma dt C + beta dt^2
+ * K(U) factored on the GPU; consistent mass, C = damping_mass M. */
+typedef struct {
+    int precision;           /* KSH_FP64 | KSH_FP32_IR (factor of the iteration matrix)    */
+    int nsteps;
+    double dt;               /* "DT"                                                      */
+    double beta, gamma;      /* the reference's "alpha" (0.25) and "delta" (0.5)           */
+    double damping_mass;     /* C = damping_mass M                                        */
+    int max_iter;            /* "MaxIter"                                                 */
+    double tol_f, tol_u;     /* "TolF", "TolU"                                            */
+    int max_refine;
+    double refine_tol;
+    int verbose;
+} ksh_newmark_options;
+
+typedef struct {
+    int steps_done;
+    int final_status;        /* KSH_SUCCESS | KSH_NOT_CONVERGED (a step's Newton loop)     */
+    double *time;            /* [nsteps] (optional)                                       */
+    double *unorm;           /* [nsteps] ||U||                                            */
+    int *iterations;         /* [nsteps] Newton iterations                                */
+    double *U, *V, *A;       /* [nsteps*n] states after each step (optional)              */
+    double *w;               /* [nsteps] U at probe_dof (optional)                        */
+    int probe_dof;
+    double wall_seconds;
+    int64_t jacobians;
+} ksh_newmark_result;
+
+/* U0, V0, A0: initial state [n] (NULL: zero) */
+int ksh_run_newmark(ksh_problem *pb, const ksh_newmark_options *opt, const double *U0, const double *V0,
+                    const double *A0, ksh_newmark_result *res);
+/* many problems from rest (U = V = A = 0) at once; res[count] */
+int ksh_batch_run_newmark(ksh_problem *const *pbs, int count, const ksh_newmark_options *opt, ksh_newmark_result *res,
+                          double *wall_seconds);
+
 /* ---- eigen-solvers --------------------------------------------------------------
  * Generalized symmetric eigenpairs A x = lambda B x nearest `shift`, by thick-restart Lanczos
  * on a shift-invert operator whose solves use the GPU band factor of A - shift B.  Replaces

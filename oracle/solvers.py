@@ -37,6 +37,9 @@ class OracleOps:
         self.n_res += 1
         return self.o.residual_static(u, lam)
 
+    def mass(self):
+        return self.o.mass_band()
+
     def factor(self, ab):
         self.n_fact += 1
         f = BandLDLT(ab, self.n, self.bw)
@@ -566,3 +569,139 @@ def run_static_newton(patch, nsteps, dlambda, max_iter=25, tol=1e-10):
     r = {key: np.array(v) for key, v in out.items()}
     r.update(steps_done=len(out["lambda"]), wall_seconds=time.perf_counter() - t0, jacobians=ops.n_jac)
     return r
+
+
+# ---------------------------------------------------------------------------------------------
+def lumped_mass(ops):
+    """assembleMass(true): the row sums of the consistent mass matrix (the B-spline basis is a
+    partition of unity, so row i sums to the integral of rho t N_i)."""
+    n = ops.n
+    return band_symv(ops.mass(), n, ops.bw, np.ones(n))
+
+
+def run_dynamic_relaxation(patch, ops=None, damping=1.0, alpha=2.0, tol=1e-6, tol_f=-1.0, tol_u=-1.0, tol_e=1e-6,
+                           max_it=25, reset_it=-1, mass=None):
+    """gsStaticDR::_solve (gsStaticDR.hpp:93-137) with _start (:219-266), _iteration (:182-201) and
+    _peak (:204-214), dt = 1, the static residual F - F_int(U) at the full load, lumped mass.
+    Returns a dict: U, iterations, converged, residual (||R|| / ||R0||), ek history, peaks."""
+    ops = ops if ops is not None else OracleOps(patch)
+    n = ops.n
+    m = lumped_mass(ops) if mass is None else np.asarray(mass, dtype=np.float64)
+    minv = 1.0 / m / alpha
+    damp = damping * m
+    tolF = tol_f if tol_f != -1 else tol
+    tolU = tol_u if tol_u != -1 else tol
+    dt = 1.0
+    F = ops.F
+
+    def res(u):
+        return ops.residual_static(u, 1.0)
+
+    U = np.zeros(n)
+    # _start (no headstart)
+    v = np.zeros(n)
+    DU = np.zeros(n)
+    R = res(U)
+    residual = np.linalg.norm(F)
+    if residual == 0:
+        residual = 1.0
+    residual_ini = residual
+    dU = 0.5 * dt * dt * minv * R
+    v = dU / dt
+    DU = DU + dU
+    Ek = v @ (m * v)
+    Ek0 = Ek
+    eks = [Ek]
+    peaks = 0
+    converged = False
+    it = 1
+    reset = 0
+    while it != max_it:
+        # _iteration
+        Ek_prev = Ek
+        R = res(U + DU) - damp * v
+        residual = np.linalg.norm(R)
+        if residual_ini == 0:
+            residual_ini = residual
+        v = v + dt * minv * R
+        dU = dt * v
+        DU = DU + dU
+        Ek = v @ (m * v)
+        if (damping == 0 and Ek_prev > Ek) or reset == reset_it:
+            reset = 0
+            # _peak
+            R = res(U + DU) - damp * v
+            dU = -1.5 * dt * v + dt * dt / 2.0 * minv * R
+            DU = DU + dU
+            v = np.zeros(n)
+            Ek = 0.0
+            peaks += 1
+        eks.append(Ek)
+        if residual / residual_ini < tolF and Ek / Ek0 < tol_e and np.linalg.norm(dU) / np.linalg.norm(DU) < tolU:
+            U = U + DU
+            converged = True
+            break
+        if it == max_it - 1:
+            U = U + DU
+            break
+        it += 1
+        reset += 1
+    return {"U": U, "iterations": it, "converged": converged, "residual": residual / residual_ini,
+            "ek": np.array(eks), "peaks": peaks, "residuals": ops.n_res}
+
+
+def run_newmark(patch, nsteps, dt, ops=None, beta=0.25, gamma=0.5, damping_mass=0.0, max_iter=100, tol_f=1e-3,
+                tol_u=1e-6, U0=None, V0=None, A0=None):
+    """gsDynamicNewmark<T, true>::_step_impl (gsDynamicNewmark.hpp:69-142) under the time loop of
+    tutorials/nonlinear_shell_dynamic.cpp:176-186: per step the predictor with A = 0, then Newton
+    on M A + C V = R(U) with lhs M + gamma dt C + beta dt^2 K(U); consistent mass, C =
+    damping_mass M, R = F - F_int.  ("alpha" and "delta" of the reference are beta and gamma.)"""
+    ops = ops if ops is not None else OracleOps(patch)
+    n = ops.n
+    M = ops.mass()
+    bw = ops.bw
+    U = np.zeros(n) if U0 is None else np.array(U0, dtype=np.float64)
+    V = np.zeros(n) if V0 is None else np.array(V0, dtype=np.float64)
+    A = np.zeros(n) if A0 is None else np.array(A0, dtype=np.float64)
+    out = {"time": [], "U": [], "V": [], "A": [], "iterations": [], "unorm": []}
+    t = 0.0
+    status = 0
+    for k in range(nsteps):
+        Uold, Vold, Aold = U, V, A
+        A = np.zeros(n)
+        U = Uold + dt * Vold + Aold * (0.5 - beta) * dt * dt + beta * dt * dt * A
+        V = Vold + Aold * (1 - gamma) * dt + gamma * dt * A
+        R = ops.residual_static(U, 1.0)
+        rhs = R - damping_mass * band_symv(M, n, bw, V) - band_symv(M, n, bw, A)
+        r0 = np.linalg.norm(rhs)
+        r0 = r0 if r0 != 0 else 1.0
+        ok = False
+        for it in range(max_iter):
+            K = ops.jacobian(U)
+            lhs = (1.0 + gamma * dt * damping_mass) * M + dt * dt * beta * K
+            fac = ops.factor(lhs)
+            dA = ops.solve(fac, rhs)
+            A = A + dA
+            V = V + dA * gamma * dt
+            U = U + dA * beta * dt * dt
+            an, dan = np.linalg.norm(A), np.linalg.norm(dA)
+            upd = dan / an if an != 0 else dan
+            R = ops.residual_static(U, 1.0)
+            rhs = R - damping_mass * band_symv(M, n, bw, V) - band_symv(M, n, bw, A)
+            rn = np.linalg.norm(rhs) / r0
+            if upd < tol_u and rn < tol_f:
+                ok = True
+                break
+        if not ok:
+            status = 1
+            break
+        t += dt
+        out["time"].append(t)
+        out["U"].append(U.copy())
+        out["V"].append(V.copy())
+        out["A"].append(A.copy())
+        out["iterations"].append(it + 1)
+        out["unorm"].append(np.linalg.norm(U))
+    res = {key: np.array(v) for key, v in out.items()}
+    res.update(steps_done=len(out["time"]), status=status, jacobians=ops.n_jac)
+    return res
