@@ -1,5 +1,5 @@
-"""Captured factor launch sequences (hipGraph replays) against the same factorizations run
-eagerly (KSH_GRAPHS=0): every replay, with the program's own eager work and synchronizations
+"""Captured factor launch sequences (hipGraph replays, opt-in with KSH_GRAPHS=1) against the same
+factorizations run eagerly (the default, KSH_GRAPHS=0): every replay, with the program's own eager work and synchronizations
 between replays, must give bit-identical pivots, solutions and paths."""
 import numpy as np
 import pytest
