@@ -184,6 +184,38 @@ def limit_points(sps, res):
     return out
 
 
+def c4_leg():
+    """C4: the 10 lowest modal and buckling pairs of the 128 x 128 degree-3 plate (49,411 free
+    dofs) in FP64 and with the FP32 factor + FP64 refinement: seconds to 10 converged pairs, the
+    solves, and the largest relative eigenvalue error against ARPACK on the oracle's matrices
+    (tests/golden/c4_eigen.json)."""
+    from gsstructuralanalysis_amd import _lib as L
+    from gsstructuralanalysis_amd import geometry as g
+    from gsstructuralanalysis_amd.problem import ShellProblem
+    with open(os.path.join(ROOT, "tests", "golden", "c4_eigen.json")) as f:
+        fx = json.load(f)
+
+    def z0(x, y):
+        return 0.05 * np.sin(np.pi * x) * np.sin(np.pi * y)
+
+    sp = ShellProblem(g.clamped_plate_problem(p=3, n=128, z0=z0, rho=1e3))
+    out = {"workload": "C4: clamped plate 1 x 1, t 0.01, E 1e7, nu 0.3, rho 1e3, initial deflection "
+                       "0.05 sin(pi x) sin(pi y), degree 3, 128 x 128 elements",
+           "nfree": sp.nfree, "band": sp.bw}
+    for name, run, key in (("modal", sp.run_modal, "modal_omega2"), ("buckling", sp.run_buckling, "buckling_mu")):
+        ref = np.sort(np.asarray(fx[key]))
+        for form, kw in (("fp64", dict(precision=L.FP64)), ("fp32_ir", dict(precision=L.FP32_IR))):
+            t0 = time.perf_counter()
+            r = run(10, tol=1e-10, **kw)
+            dt = time.perf_counter() - t0
+            err = float(np.max(np.abs(np.sort(r["values"]) - ref) / np.abs(ref)))
+            out[f"{name}_{form}"] = {"seconds": round(dt, 3), "converged": int(r["converged"]), "solves": int(r["solves"]),
+                                     "max_rel_error_vs_arpack": err}
+            log(f"[bench] C4 {name} {form}: {dt:.2f} s, {r['solves']} solves, max rel. error vs ARPACK {err:.1e}")
+    sp.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,6 +227,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=24.0, help="seconds of CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 reference timing")
+    ap.add_argument("--no-c4", action="store_true", help="skip the C4 eigen-analysis leg")
     ap.add_argument("--plain-sample", type=int, default=2, help="plain FP64: instances timed per size class")
     ap.add_argument("--plain-all", action="store_true", help="plain FP64: time every instance")
     ap.add_argument("--form", choices=["mixed+learned", "mixed", "batched+learned", "batched"], default="mixed+learned",
@@ -446,6 +479,10 @@ def main():
         cpu = cpu_port(sps, by_size, steps, lengths, args.cpu_budget)
         log(f"[bench] CPU port: {cpu['value']} steps/s")
 
+    c4 = None
+    if not args.no_c4 and not args.only:
+        c4 = c4_leg()
+
     amortised = None
     if training is not None and args.form.endswith("+learned"):
         amortised = round(total_steps / (t_head + training["data_seconds"] + training["train_seconds"]), 3)
@@ -491,6 +528,7 @@ def main():
         "autotuner": tuned,
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "c4": c4,
     }
     print(json.dumps(line), flush=True)
 

@@ -48,21 +48,26 @@ __global__ __launch_bounds__(kSB) void k_band_fwd_step(const T *fac, int n, int 
         l[jj] = load_if<T>(ok, rs, ((unsigned)(c0 + jj) * ldu + (unsigned)(lane - jj)) * es);
     }
     const T d = load_if<T>(lane < w, rs, (unsigned)(c0 + lane) * ldu * es);
+    // this workgroup's rows c0+r, r = w + 64 blockIdx + lane: their band entries do not depend on
+    // the diagonal block's solution, so they are loaded together with it
+    const int r = w + kSB * (int)blockIdx.x + lane;
+    T u[kSB];
+#pragma unroll
+    for (int kk = 0; kk < kSB; ++kk) {
+        const bool ok = kk < w && r < nr && r - kk <= bw;
+        u[kk] = load_if<T>(ok, rs, ((unsigned)(c0 + kk) * ldu + (unsigned)(r - kk)) * es);
+    }
+    const double rb = r < nr ? b[c0 + r] : 0.0;
     double y = lane < w ? b[c0 + lane] : 0.0;
 #pragma unroll
     for (int jj = 0; jj < kSB - 1; ++jj) y -= (double)l[jj] * readlane_f64(y, jj);  // l = 0 on and above the diagonal
     ys[lane] = y;
     if (blockIdx.x == 0 && lane < w) z[c0 + lane] = y / (double)d;
     __syncthreads();
-    // rows c0+r, r = w + 64 blockIdx + lane
-    const int r = w + kSB * (int)blockIdx.x + lane;
     double acc = 0.0;
-#pragma unroll 16
-    for (int kk = 0; kk < kSB; ++kk) {
-        const bool ok = kk < w && r < nr && r - kk <= bw;
-        acc += (double)load_if<T>(ok, rs, ((unsigned)(c0 + kk) * ldu + (unsigned)(r - kk)) * es) * ys[kk];
-    }
-    if (r < nr) b[c0 + r] -= acc;
+#pragma unroll
+    for (int kk = 0; kk < kSB; ++kk) acc += (double)u[kk] * ys[kk];
+    if (r < nr) b[c0 + r] = rb - acc;
 }
 
 template <typename T>
@@ -81,19 +86,22 @@ __global__ __launch_bounds__(kSB) void k_band_bwd_step(const T *fac, int n, int 
         const bool ok = kk > lane && kk < w && kk - lane <= bw;
         l[kk] = load_if<T>(ok, rs, ((unsigned)(c0 + lane) * ldu + (unsigned)(kk - lane)) * es);
     }
+    // rows i = c0 - 1 - (64 blockIdx + q), q = 0..63, above the block within the band:
+    // z[i] -= sum_k L[c0+k][i] x_k; lane = k.  Their entries are loaded with the diagonal block.
+    const int lo = max(0, c0 - bw);
+    T u[kSB];
+#pragma unroll
+    for (int q = 0; q < kSB; ++q) {
+        const int i = c0 - 1 - (kSB * (int)blockIdx.x + q);
+        const bool ok = i >= lo && lane < w && c0 + lane - i <= bw;
+        u[q] = load_if<T>(ok, rs, ((unsigned)max(i, 0) * ldu + (unsigned)(c0 + lane - max(i, 0))) * es);
+    }
     double xv = lane < w ? z[c0 + lane] : 0.0;
 #pragma unroll
     for (int kk = kSB - 1; kk > 0; --kk) xv -= (double)l[kk] * readlane_f64(xv, kk);  // l = 0 on and below the diagonal
     if (blockIdx.x == 0 && lane < w) x[c0 + lane] = xv;
-    // rows i = c0 - 1 - (64 blockIdx + q), q = 0..63, above the block within the band:
-    // z[i] -= sum_k L[c0+k][i] x_k; lane = k
-    const int lo = max(0, c0 - bw);
-#pragma unroll 16
-    for (int q = 0; q < kSB; ++q) {
-        const int i = c0 - 1 - (kSB * (int)blockIdx.x + q);
-        const bool ok = i >= lo && lane < w && c0 + lane - i <= bw;
-        prod[q][lane] = ok ? (double)load_if<T>(ok, rs, ((unsigned)i * ldu + (unsigned)(c0 + lane - i)) * es) * xv : 0.0;
-    }
+#pragma unroll
+    for (int q = 0; q < kSB; ++q) prod[q][lane] = (double)u[q] * xv;  // masked entries loaded as 0
     __syncthreads();
     const int i = c0 - 1 - (kSB * (int)blockIdx.x + lane);
     double s = 0.0;
