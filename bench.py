@@ -237,8 +237,8 @@ def main():
                     help="time the headline form only (no other forms, no held-out predictor evaluation)")
     ap.add_argument("--also", nargs="*", default=None, metavar="FORM",
                     help="the other forms to time (default: all); compared with the FP64 batched form")
-    ap.add_argument("--max-refine", type=int, default=4, help="mixed form: Krylov (FGMRES) steps per solve with the FP32 factor as preconditioner")
-    ap.add_argument("--refine-tol", type=float, default=1e-6,
+    ap.add_argument("--max-refine", type=int, default=3, help="mixed form: Krylov (FGMRES) steps per solve with the FP32 factor as preconditioner")
+    ap.add_argument("--refine-tol", type=float, default=1e-4,
                     help="mixed form: solve stop ||b - K x|| <= tol ||b|| (the corrector's FP64 residual decides convergence)")
     ap.add_argument("--refine-accept", type=float, default=1e-2,
                     help="mixed form: a solve still above accept ||b|| after the sweeps is redone with an FP64 factor")

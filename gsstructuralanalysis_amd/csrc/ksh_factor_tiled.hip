@@ -475,12 +475,40 @@ __global__ __launch_bounds__(kPT) void k_band_panel(const BandDesc *descs, int c
         panel_general<T>(descs[blockIdx.x], c0);
 }
 
+// XCD-aware assignment of a launch's workgroups to (band, index within the band).  Workgroups are
+// dispatched to the 8 XCDs round robin (id mod 8), each XCD with its own L2.  Every trailing block
+// of a band reads that band's panel, so with 8 or more bands the workgroups of band b take the ids
+// h = b (mod 8): all of them run on one XCD and its L2 serves the panel, which otherwise every
+// XCD would fetch once per band.  The panel kernels already put band b on workgroup b, the same
+// XCD.  With fewer bands than XCDs the plain mapping keeps every XCD busy.  Returns false for
+// the padding workgroups of the XCD-aware grid (8 ceil(active / 8) wg of them).
+constexpr int kXcds = 8;
+__device__ __forceinline__ bool band_of_workgroup(int h, int active, int wg, int &band, int &local)
+{
+    if (active < kXcds) {
+        band = h / wg;
+        local = h % wg;
+    } else {
+        const int s = h / kXcds;
+        band = h % kXcds + kXcds * (s / wg);
+        local = s % wg;
+    }
+    return band < active;
+}
+inline unsigned band_grid(int active, int wg)
+{
+    return (unsigned)(active < kXcds ? active * wg : ((active + kXcds - 1) / kXcds) * kXcds * wg);
+}
+
 // One part of the trailing update at c0, kUW blocks (one per wave) per workgroup
 template <typename T>
-__global__ __launch_bounds__(64 * kUW) void k_band_trailing(const BandDesc *descs, int c0, int wg_per_band, int part)
+__global__ __launch_bounds__(64 * kUW) void k_band_trailing(const BandDesc *descs, int c0, int wg_per_band, int part,
+                                                            int active)
 {
-    const int p[1] = {__builtin_amdgcn_readfirstlane((int)((blockIdx.x % wg_per_band) * kUW + (threadIdx.x >> 6)))};
-    trailing_blocks<T, 1>(descs[blockIdx.x / wg_per_band], c0, part, p);
+    int band, local;
+    if (!band_of_workgroup((int)blockIdx.x, active, wg_per_band, band, local)) return;
+    const int p[1] = {__builtin_amdgcn_readfirstlane((int)(local * kUW + (threadIdx.x >> 6)))};
+    trailing_blocks<T, 1>(descs[band], c0, part, p);
 }
 
 // Lookahead step: workgroups [0, npanel) factor the panel at c0 + NB of their band while the others
@@ -488,8 +516,11 @@ __global__ __launch_bounds__(64 * kUW) void k_band_trailing(const BandDesc *desc
 // workgroups take NBLK blocks each, the other waves leave at once.  The two parts touch disjoint
 // columns: the panel reads and writes columns [c0 + NB, c0 + 2 NB), already updated by the kNext
 // part; the rest of the update reads the panel at c0 and writes columns from c0 + 2 NB on.
+// (The update part starts at workgroup `pad`, npanel rounded up to a multiple of the XCD count,
+// so that its XCD-aware mapping lines up with the panels' workgroups.)
 template <typename T, bool kRows, int AW, int NBLK>
-__global__ __launch_bounds__(kPT) void k_band_step(const BandDesc *descs, int c0, int npanel, int wg_per_band)
+__global__ __launch_bounds__(kPT) void k_band_step(const BandDesc *descs, int c0, int npanel, int wg_per_band,
+                                                    int pad, int active)
 {
     const int b = (int)blockIdx.x;
     if (b < npanel) {
@@ -499,13 +530,15 @@ __global__ __launch_bounds__(kPT) void k_band_step(const BandDesc *descs, int c0
             panel_general<T>(descs[b], c0 + kNB);
         return;
     }
+    if (b < pad) return;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     if (wave >= AW) return;
-    const int q = b - npanel, local = q % wg_per_band;
+    int band, local;
+    if (!band_of_workgroup(b - pad, active, wg_per_band, band, local)) return;
     int p[NBLK];
 #pragma unroll
     for (int s = 0; s < NBLK; ++s) p[s] = (local * NBLK + s) * AW + wave;
-    trailing_blocks<T, NBLK>(descs[q / wg_per_band], c0, kRest, p);
+    trailing_blocks<T, NBLK>(descs[band], c0, kRest, p);
 }
 
 // negatives, min d, max |d|, min |d| of each factored diagonal (one workgroup of 1024 threads per band)
@@ -587,12 +620,13 @@ void run_lookahead(hipStream_t s, const BandDesc *d_descs, const std::vector<int
     hipLaunchKernelGGL((k_band_panel<T, kRows>), dim3((unsigned)nbands), dim3(kPT), 0, s, d_descs, 0);
     for (int c0 = 0; c0 < ns[0]; c0 += kNB) {
         const int active = active_at(c0), np = active_at(c0 + kNB);
-        hipLaunchKernelGGL(k_band_trailing<T>, dim3((unsigned)(active * wg1)), dim3(64 * kUW), 0, s, d_descs, c0, wg1,
-                           (int)kNext);
-        const int grid = np + active * wg2;
+        hipLaunchKernelGGL(k_band_trailing<T>, dim3(band_grid(active, wg1)), dim3(64 * kUW), 0, s, d_descs, c0, wg1,
+                           (int)kNext, active);
+        const int pad = wg2 > 0 && active >= kXcds ? (np + kXcds - 1) / kXcds * kXcds : np;
+        const int grid = pad + (wg2 > 0 ? (int)band_grid(active, wg2) : 0);
         if (grid > 0)
             hipLaunchKernelGGL((k_band_step<T, kRows, AW, NBLK>), dim3((unsigned)grid), dim3(kPT), 0, s, d_descs, c0, np,
-                               wg2);
+                               wg2, pad, active);
     }
 }
 
@@ -608,8 +642,8 @@ void run_tiled_sched(hipStream_t s, const BandDesc *d_descs, const std::vector<i
         for (int c0 = 0; c0 < ns[0]; c0 += kNB) {
             while (active > 0 && ns[active - 1] <= c0) --active;
             hipLaunchKernelGGL((k_band_panel<T, kRows>), dim3((unsigned)active), dim3(kPT), 0, s, d_descs, c0);
-            hipLaunchKernelGGL(k_band_trailing<T>, dim3((unsigned)(active * wg)), dim3(64 * kUW), 0, s, d_descs, c0, wg,
-                               (int)kAll);
+            hipLaunchKernelGGL(k_band_trailing<T>, dim3(band_grid(active, wg)), dim3(64 * kUW), 0, s, d_descs, c0, wg,
+                               (int)kAll, active);
         }
     } else if (sched == 1) {
         run_lookahead<T, kRows, 8, 1>(s, d_descs, ns, maxbw);

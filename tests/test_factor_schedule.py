@@ -151,3 +151,39 @@ def test_lookahead_reads_what_plain_reads(ns, bws):
     for u in s0:
         assert s0[u] == s1[u], u
     assert l0 == l1
+
+
+# ---- XCD-aware workgroup -> (band, index) mapping of the trailing launches ---------------------
+XCDS = 8
+
+
+def band_of_workgroup(h, active, wg):
+    """The kernels' mapping (ksh_factor_tiled.hip, band_of_workgroup)."""
+    if active < XCDS:
+        band, local = h // wg, h % wg
+    else:
+        s = h // XCDS
+        band, local = h % XCDS + XCDS * (s // wg), s % wg
+    return band, local, band < active
+
+
+def band_grid(active, wg):
+    return active * wg if active < XCDS else -(-active // XCDS) * XCDS * wg
+
+
+@pytest.mark.parametrize("active", [1, 3, 7, 8, 9, 16, 37, 256])
+@pytest.mark.parametrize("wg", [1, 2, 5])
+@pytest.mark.parametrize("npanel", [0, 5, 8, 13])
+def test_xcd_mapping_covers_every_band_workgroup_once(active, wg, npanel):
+    """Every (band, index) of a launch is taken by exactly one workgroup, the rest exit; with 8 or
+    more bands a band's workgroups all sit on the XCD of its panel workgroup (id = band mod 8),
+    also behind the panel workgroups of a lookahead step launch (offset padded to a multiple of 8)."""
+    pad = -(-npanel // XCDS) * XCDS if active >= XCDS else npanel
+    seen = []
+    for h in range(pad, pad + band_grid(active, wg)):
+        b, l, ok = band_of_workgroup(h - pad, active, wg)
+        if ok:
+            seen.append((b, l))
+            if active >= XCDS:
+                assert h % XCDS == b % XCDS
+    assert sorted(seen) == [(b, l) for b in range(active) for l in range(wg)]
