@@ -206,38 +206,34 @@ int inertia_fp64(Matrix &K, int64_t *neg, double *dmin)
 
 int refactor_fp64(Matrix &K) { return inertia_fp64(K, nullptr, nullptr); }
 
-// Three power steps z <- z - M^-1 K z = M^-1 E z with the FP32 factor M = K + E of K, from a fixed
-// pseudo-random start; the contraction of the last step estimates rho(M^-1 E), the quantity the
-// pivot-count certificate compares with the guard.
+// kPowerSteps power steps z <- z - M^-1 K z = M^-1 E z with the FP32 factor M = K + E of K, from the
+// fixed pseudo-random start; the contraction of the last step estimates rho(M^-1 E), the quantity
+// the pivot-count certificate compares with the guard.
 int power_contraction(Matrix &K)
 {
     hipStream_t s = K.pb->stream;
     const int n = K.n;
     double *z = K.work + 2 * (size_t)n, *y = K.work + 3 * (size_t)n;
     std::vector<double> h(n);
-    uint64_t st = 0x9E3779B97F4A7C15ull;
-    for (int i = 0; i < n; ++i) {
-        st = st * 6364136223846793005ull + 1442695040888963407ull;
-        h[i] = (double)(st >> 11) * (1.0 / 9007199254740992.0) - 0.5;
-    }
+    power_start_vector(h.data(), (size_t)n);
     KSH_HIP(hipMemcpyAsync(z, h.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
     int rc;
     double prev = 0.0;
     if ((rc = vec_norm2(s, n, z, K.red, &prev))) return rc;
-    for (int k = 0; k < 3; ++k) {
+    K.rho_pw = 0.0;
+    for (int k = 0; k < kPowerSteps; ++k) {
         if ((rc = band_symv(K, z, y)) || (rc = band_solve(K, K.fac, KSH_FP32, y)) || (rc = vec_sub(s, n, z, y, z))) return rc;
         double nz = 0.0;
         if ((rc = vec_norm2(s, n, z, K.red, &nz))) return rc;
         const double ratio = prev > 0 ? nz / prev : 0.0;
         if (!std::isfinite(ratio)) {
-            K.rho = INFINITY;
+            K.rho_pw = INFINITY;
             break;
         }
-        K.rho = ratio;  // the last step's contraction
+        K.rho_pw = ratio;  // the last step's contraction
         prev = nz;
         if (nz == 0) break;
     }
-    K.rho_obs = 3;
     return KSH_SUCCESS;
 }
 
@@ -650,6 +646,7 @@ int ksh_factor(ksh_matrix *K)
     K->use64 = false;
     K->rho = 0.0;
     K->rho_obs = 0;
+    K->rho_pw = -1.0;
     double st[4] = {0, 0, 0, 0};
     int rc = band_factor(*K, K->fac, K->fprec, st);
     K->factored = true;
@@ -736,8 +733,8 @@ int ksh_inertia(ksh_matrix *K, int64_t *n_negative, double *d_min)
     int rc;
     if ((rc = ksh_factor(K))) return rc;
     if (K->fprec == KSH_FP32 && !K->use64 && K->guard > 0) {
-        if (K->rho_obs == 0 && (rc = power_contraction(*K))) return rc;
-        if (K->rho > K->guard && (rc = inertia_fp64(*K, nullptr, nullptr))) return rc;
+        if (K->rho_pw < 0 && (rc = power_contraction(*K))) return rc;
+        if (std::max(K->rho, K->rho_pw) > K->guard && (rc = inertia_fp64(*K, nullptr, nullptr))) return rc;
     }
     int64_t neg = 0;
     double dmin = 0, dmaxabs = 0, dminabs = 0;

@@ -110,6 +110,99 @@ int BatchDevice::assemble(const std::vector<TangentJob> &jobs)
     return KSH_SUCCESS;
 }
 
+int BatchDevice::power_contraction(const std::vector<Matrix *> &ms)
+{
+    std::vector<Matrix *> todo;
+    for (Matrix *K : ms)
+        if (K->fprec == KSH_FP32 && !K->use64 && K->rho_pw < 0) todo.push_back(K);
+    if (todo.empty()) return KSH_SUCCESS;
+    const size_t nj = todo.size();
+    size_t tot = 0, nmax = 0;
+    std::vector<size_t> off(nj);
+    for (size_t i = 0; i < nj; ++i) {
+        off[i] = tot;
+        tot += (size_t)todo[i]->n;
+        nmax = std::max(nmax, (size_t)todo[i]->n);
+    }
+    int rc;
+    if (nmax > m_start_n) {
+        if ((rc = m_start.ensure(nmax * sizeof(double)))) return rc;
+        std::vector<double> h(nmax);
+        power_start_vector(h.data(), nmax);
+        KSH_HIP(hipMemcpyAsync(m_start.p, h.data(), nmax * sizeof(double), hipMemcpyHostToDevice, m_stream));
+        KSH_HIP(hipStreamSynchronize(m_stream));
+        m_start_n = nmax;
+    }
+    if ((rc = m_pw.ensure(3 * tot * sizeof(double)))) return rc;
+    double *base = (double *)m_pw.p;
+    auto Z = [&](size_t i) { return base + off[i]; };
+    auto T = [&](size_t i) { return base + tot + off[i]; };
+    auto W = [&](size_t i) { return base + 2 * tot + off[i]; };
+    std::vector<VecDesc> vd(nj);
+    std::vector<DotDesc> dd(nj);
+    for (size_t i = 0; i < nj; ++i) {
+        vd[i].n = todo[i]->n;
+        vd[i].add(Z(i), 1.0, (const double *)m_start.p);
+        dd[i].n = todo[i]->n;
+        dd[i].npairs = 1;
+        dd[i].a[0] = dd[i].b[0] = Z(i);
+    }
+    std::vector<double> prev, cur;
+    if ((rc = vecops(vd)) || (rc = dots(dd, prev))) return rc;
+    for (size_t i = 0; i < nj; ++i) todo[i]->rho_pw = 0.0;
+    std::vector<int> live(nj, 1);
+    for (int k = 0; k < kPowerSteps; ++k) {
+        std::vector<ProductJob> pj;
+        std::vector<SolveJob> sj;
+        std::vector<size_t> idx;
+        for (size_t i = 0; i < nj; ++i) {
+            if (!live[i]) continue;
+            pj.push_back(ProductJob{todo[i], Z(i), nullptr, T(i)});                            // t = -K z
+            sj.push_back(SolveJob{todo[i], T(i), W(i), 0, 0.0, KSH_SUCCESS, nullptr});       // w = -M^-1 K z
+            idx.push_back(i);
+        }
+        if (idx.empty()) break;
+        std::vector<double> rr;
+        if ((rc = products(pj, rr)) || (rc = solve(sj))) return rc;
+        for (const SolveJob &q : sj)
+            if (q.rc != KSH_SUCCESS) return q.rc;
+        std::vector<VecDesc> up;
+        std::vector<DotDesc> nd;
+        for (size_t i : idx) {
+            VecDesc d;
+            d.n = todo[i]->n;
+            d.add(Z(i), 1.0, Z(i), 1.0, W(i));  // z - M^-1 K z = M^-1 E z
+            up.push_back(d);
+            nd.push_back(dd[i]);
+        }
+        if ((rc = vecops(up)) || (rc = dots(nd, cur))) return rc;
+        std::vector<VecDesc> sc;
+        for (size_t q = 0; q < idx.size(); ++q) {
+            const size_t i = idx[q];
+            const double ratio = prev[i] > 0 ? std::sqrt(cur[q] / prev[i]) : 0.0;
+            Matrix &K = *todo[i];
+            if (!std::isfinite(ratio)) {
+                K.rho_pw = INFINITY;
+                live[i] = 0;
+                continue;
+            }
+            K.rho_pw = ratio;  // the last step's contraction
+            if (cur[q] == 0) {
+                live[i] = 0;
+                continue;
+            }
+            VecDesc d;  // rescale to unit length
+            d.n = K.n;
+            d.add(Z(i), 1.0 / std::sqrt(cur[q]), Z(i));
+            sc.push_back(d);
+            prev[i] = 1.0;
+        }
+        if ((rc = vecops(sc))) return rc;
+    }
+    prof_count("power_contraction", (long long)nj);
+    return KSH_SUCCESS;
+}
+
 int BatchDevice::products(std::vector<ProductJob> &jobs, std::vector<double> &rr)
 {
     rr.assign(jobs.size(), 0.0);
@@ -199,6 +292,7 @@ int BatchDevice::factor_matrices(const std::vector<Matrix *> &ms)
             K.info = info[k];
             K.rho = 0.0;
             K.rho_obs = 0;
+            K.rho_pw = -1.0;
             for (int q = 0; q < 4; ++q) K.fstats[q] = st[4 * k + q];
             K.has_stats = info[k] == 0;
             if (fp == KSH_FP32 && info[k] != 0) re.push_back(&K);

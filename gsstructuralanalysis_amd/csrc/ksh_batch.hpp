@@ -73,6 +73,9 @@ public:
     // FP64 refactorization of the given FP32-factored matrices (their pivot count and solves then
     // use it); matrices already on FP64 are skipped
     int refactor64(const std::vector<Matrix *> &ms);
+    // Matrix::rho_pw of the given FP32-factored matrices that lack it: kPowerSteps batched power
+    // steps z <- z - M^-1 K z from the fixed pseudo-random start
+    int power_contraction(const std::vector<Matrix *> &ms);
 
     int dots(const std::vector<DotDesc> &d, std::vector<double> &host_out);  // npairs doubles each, in order
     int vecops(const std::vector<VecDesc> &d);
@@ -101,7 +104,8 @@ private:
     void *m_hearly = nullptr;  // pinned: info + stats of the early factors
     size_t m_hearly_bytes = 0;
     DevScratch m_desc, m_bdesc, m_cdesc, m_rdesc, m_stats, m_info, m_norms, m_part, m_r, m_dotdesc, m_vecdesc, m_dotout;
-    DevScratch m_zero, m_ppart;
+    DevScratch m_zero, m_ppart, m_pw, m_start;
+    size_t m_start_n = 0;
     size_t m_zero_n = 0;
     long long m_refactors = 0;
 };

@@ -106,6 +106,20 @@ struct Problem {
     InstDesc *d_desc = nullptr;    // descriptor of this problem for single-instance launches
 };
 
+// Pivot-count certificate of FP32 factors: power steps z <- z - M^-1 K z = M^-1 E z (M = K + E the
+// matrix whose exact LDL^T the FP32 factor is) from a fixed pseudo-random start estimate
+// rho(M^-1 E).  A start with a component along every eigenvector lets the steps see a mode that
+// the solves' right-hand sides barely touch (the near-null mode at a singular point).
+constexpr int kPowerSteps = 3;
+inline void power_start_vector(double *h, size_t n)
+{
+    uint64_t st = 0x9E3779B97F4A7C15ull;
+    for (size_t i = 0; i < n; ++i) {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        h[i] = (double)(st >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    }
+}
+
 struct Matrix {
     Problem *pb = nullptr;
     int n = 0, bw = 0, prec = KSH_FP64;
@@ -129,6 +143,8 @@ struct Matrix {
     double accept = 1e-6;          // refined FP32 solves above accept ||b|| are redone in FP64
     double rho = 0.0;              // largest residual contraction seen in refinement sweeps of this
     int rho_obs = 0;               // FP32 factor (estimate of rho(M^-1 E), M = K + E) and their count
+    double rho_pw = -1.0;          // power-step estimate of rho(M^-1 E) for the current FP32 factor
+                                   // (-1: not taken); the count certificate uses max(rho, rho_pw)
     double fstats[4] = {0, 0, 0, 0};  // pivot statistics of the current factor (negatives, min d, max|d|, min|d|)
     bool has_stats = false;
     // device workspace for solves and inertia

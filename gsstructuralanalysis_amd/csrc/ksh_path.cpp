@@ -821,12 +821,16 @@ private:
     {
         int rc;
         // FP32 pivot counts without a certificate are taken from an FP64 factor of the same tangent
-        std::vector<Matrix *> recount;
+        // (certificate: the estimate of rho(M^-1 E) below the guard, see ksh_batch.cpp)
+        std::vector<Matrix *> cert, recount;
         for (Inst *I : conv) {
             Matrix &K = *I->K;
-            if (m_o.bifurcation_method < 0 || K.fprec != KSH_FP32 || K.use64) continue;
-            if (K.guard > 0 && (K.rho_obs == 0 || K.rho > K.guard)) recount.push_back(&K);
+            if (m_o.bifurcation_method < 0 || K.fprec != KSH_FP32 || K.use64 || !(K.guard > 0)) continue;
+            cert.push_back(&K);
         }
+        if ((rc = m_dev.power_contraction(cert))) return rc;
+        for (Matrix *K : cert)
+            if (std::max(K->rho, K->rho_pw) > K->guard) recount.push_back(K);
         if (!recount.empty()) {
             prof_count("fp64_refactor_count", (long long)recount.size());
             if ((rc = m_dev.refactor64(recount))) return rc;
@@ -1626,6 +1630,15 @@ private:
             }
         }
         if ((rc = m_dev.vecops(fin)) || (rc = m_dev.synchronize())) return rc;
+        {  // the recorded pivot count: FP32 counts only under the certificate, else an FP64 factor
+            std::vector<Matrix *> cert, recount;
+            for (StaticInst *I : done)
+                if (I->K->fprec == KSH_FP32 && !I->K->use64 && I->K->guard > 0) cert.push_back(I->K);
+            if ((rc = m_dev.power_contraction(cert))) return rc;
+            for (Matrix *K : cert)
+                if (std::max(K->rho, K->rho_pw) > K->guard) recount.push_back(K);
+            if (!recount.empty() && (rc = m_dev.refactor64(recount))) return rc;
+        }
         for (StaticInst *I : done) {
             ksh_static_result &r = *I->res;
             const int k = I->k;
