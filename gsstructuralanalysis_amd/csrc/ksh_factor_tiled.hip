@@ -60,39 +60,39 @@ constexpr int kXr = kNS + 1;  // row stride of the panel update's per-wave LDS r
 // diagonal take updates too; never read.  On return blk holds L below the diagonal and D on it;
 // dval / dinv the pivots and their inverses.  The first bad pivot (zero, not finite, or not
 // finite in the storage precision T) is recorded in *info; `clear` resets it first.
-template <typename T>
-__device__ __forceinline__ void diag_factor(double *blk, double *dval, double *dinv, int w, int col0, int *info, bool clear)
+template <typename T, typename C>
+__device__ __forceinline__ void diag_factor(C *blk, C *dval, C *dinv, int w, int col0, int *info, bool clear)
 {
     const int lane = threadIdx.x & 63;
     const int r = lane & (kNS - 1);
-    double a[kNS];
+    C a[kNS];
 #pragma unroll
     for (int kk = 0; kk < kNS; ++kk) a[kk] = blk[kk * kNS + r];
     int bad = 0;
 #pragma unroll
     for (int jj = 0; jj < kNS; ++jj) {
-        const double cv = a[jj];  // A[r][jj]; lane kk's value is A[kk][jj]
+        const C cv = a[jj];  // A[r][jj]; lane kk's value is A[kk][jj]
         // the whole column first, into distinct scalar registers: the reads issue back to back and
         // the reciprocal overlaps them, instead of a read / wait / multiply-add triple per entry
-        double col[kNS];
+        C col[kNS];
 #pragma unroll
-        for (int kk = jj; kk < kNS; ++kk) col[kk] = readlane_f64(cv, kk);
-        const double djj = col[jj];
-        if (jj < w && bad == 0 && (djj == 0.0 || !isfinite(djj) || !isfinite((double)(T)djj))) bad = col0 + jj + 1;
-        const double l = cv * (1.0 / djj);
+        for (int kk = jj; kk < kNS; ++kk) col[kk] = bcast(cv, kk);
+        const C djj = col[jj];
+        if (jj < w && bad == 0 && (djj == C(0) || !isfinite(djj) || !isfinite((double)(T)djj))) bad = col0 + jj + 1;
+        const C l = cv * (C(1) / djj);
 #pragma unroll
         for (int kk = jj + 1; kk < kNS; ++kk) a[kk] -= l * col[kk];
         if (r > jj) a[jj] = l;
         __builtin_amdgcn_sched_barrier(0);
     }
     if (lane == 0 && info && (clear || (bad != 0 && *info == 0))) *info = bad;
-    double ar = 0.0;  // a[r] by selects: a lane-dependent index would put a[] in scratch memory
+    C ar = C(0);  // a[r] by selects: a lane-dependent index would put a[] in scratch memory
 #pragma unroll
     for (int kk = 0; kk < kNS; ++kk) ar = kk == r ? a[kk] : ar;
     dval[r] = ar;
-    dinv[r] = 1.0 / ar;
+    dinv[r] = C(1) / ar;
 #pragma unroll
-    for (int kk = 0; kk < kNS; ++kk) blk[kk * kNS + r] = kk <= r ? a[kk] : 0.0;
+    for (int kk = 0; kk < kNS; ++kk) blk[kk * kNS + r] = kk <= r ? a[kk] : C(0);
 }
 
 // Sub-panel at column cs (width ws = min(32, n - cs)): factor the diagonal block in LDS (wave 0),
@@ -119,7 +119,7 @@ __device__ __forceinline__ void sub_panel(const BandDesc &d, __amdgpu_buffer_rsr
         blk[idx] = (r >= w && r == kk) ? 1.0 : v;
     }
     __syncthreads();
-    if (wave == 0) diag_factor<T>(blk, dval, dinv, w, cs, d.info, cs == 0);
+    if (wave == 0) diag_factor<T, double>(blk, dval, dinv, w, cs, d.info, cs == 0);
     __syncthreads();
     // write the factored block back (L below the diagonal, D on it)
 #pragma unroll
@@ -205,10 +205,12 @@ __device__ __forceinline__ void panel_general(const BandDesc &d, int c0)
 template <typename T>
 __device__ __forceinline__ void panel_rows(const BandDesc &d, int c0)
 {
-    __shared__ double blk1[kNS * kNS], blk2[kNS * kNS];
-    __shared__ double dval1[kNS], dinv1[kNS], dval2[kNS], dinv2[kNS];
-    __shared__ double lnext[kNS][kNS + 1];  // L[c0+32+j][c0+k] of the first sub-panel
-    __shared__ double xr[kPT / 64][64 * kXr];  // per wave: its 64 rows, first L d, then the update
+    using C = T;  // arithmetic in the storage precision
+    using M = Mfma16<C>;
+    __shared__ C blk1[kNS * kNS], blk2[kNS * kNS];
+    __shared__ C dval1[kNS], dinv1[kNS], dval2[kNS], dinv2[kNS];
+    __shared__ C lnext[kNS][kNS + 1];  // L[c0+32+j][c0+k] of the first sub-panel
+    __shared__ C xr[kPT / 64][64 * kXr];  // per wave: its 64 rows, first L d, then the update
     const int n = d.n, bw = d.bw;
     if (c0 >= n) return;
     T *fac = (T *)d.fac;
@@ -226,19 +228,19 @@ __device__ __forceinline__ void panel_rows(const BandDesc &d, int c0)
     for (int q = 0; q < kQ; ++q) {
         const int idx = tid + kPT * q, kk = idx / kNS, rr = idx % kNS;
         const bool ok = rr >= kk && rr < w1 && rr - kk <= bw;
-        const double v = (double)load_if<T>(ok, rs, ((unsigned)(c0 + kk) * ldu + (unsigned)(rr - kk)) * es);
-        blk1[idx] = (rr >= w1 && rr == kk) ? 1.0 : v;
+        const C v = load_if<T>(ok, rs, ((unsigned)(c0 + kk) * ldu + (unsigned)(rr - kk)) * es);
+        blk1[idx] = (rr >= w1 && rr == kk) ? C(1) : v;
     }
     __syncthreads();
-    if (wave == 0) diag_factor<T>(blk1, dval1, dinv1, w1, c0, d.info, c0 == 0);
+    if (wave == 0) diag_factor<T, C>(blk1, dval1, dinv1, w1, c0, d.info, c0 == 0);
     __syncthreads();
     // the row is loaded in two halves, each just before it is needed, so that no more than 64
     // of its values are in registers at once (and none in the factoring wave during a factor)
-    double x[kNB];
+    C x[kNB];
 #pragma unroll
     for (int k = 0; k < kNS; ++k) {
         const bool ok = has_row && k < w && r - k <= bw;
-        x[k] = (double)load_if<T>(ok, rs, ((unsigned)(c0 + k) * ldu + (unsigned)(r - k)) * es);
+        x[k] = load_if<T>(ok, rs, ((unsigned)(c0 + k) * ldu + (unsigned)(r - k)) * es);
     }
     // first diagonal block back to the band
 #pragma unroll
@@ -250,7 +252,7 @@ __device__ __forceinline__ void panel_rows(const BandDesc &d, int c0)
     // TRSM of the first sub-panel: L[r][0:32] = A[r][0:32] (D L_dd^T)^-1
 #pragma unroll
     for (int kk = 0; kk < kNS; ++kk) {
-        const double tk = x[kk];
+        const C tk = x[kk];
 #pragma unroll
         for (int jj = kk + 1; jj < kNS; ++jj) x[jj] -= tk * blk1[kk * kNS + jj];
         x[kk] = tk * dinv1[kk];
@@ -268,7 +270,7 @@ __device__ __forceinline__ void panel_rows(const BandDesc &d, int c0)
 #pragma unroll
     for (int k = kNS; k < kNB; ++k) {
         const bool ok = has_row && k < w && r >= k && r - k <= bw;
-        x[k] = (double)load_if<T>(ok, rs, ((unsigned)(c0 + k) * ldu + (unsigned)(r - k)) * es);
+        x[k] = load_if<T>(ok, rs, ((unsigned)(c0 + k) * ldu + (unsigned)(r - k)) * es);
     }
     if (r < kNB) {
 #pragma unroll
@@ -276,33 +278,34 @@ __device__ __forceinline__ void panel_rows(const BandDesc &d, int c0)
     }
     __syncthreads();
     // rank-32 update of the second part, x[32+j] -= S[r][j], S[r][j] = sum_k L[r][k] d_k L[32+j][k],
-    // on FP64 MFMA per wave (entries above the diagonal of rows 32..63 are updated too; never
+    // on MFMA per wave (entries above the diagonal of rows 32..63 are updated too; never
     // read).  The wave's rows go through its LDS region: row i of the wave holds L[r][k] d_k at
     // xr[i*kXr + k], then S[r][j] at xr[i*kXr + j].  S^T tiles (16 j x 16 rows): A = L[32+j][k]
     // from lnext, B = (L d)^T from the region, so that each value is read once per lane (a
     // loop over rows with lnext broadcast to every lane costs 32 x 32 LDS broadcasts per wave).
     {
-        double *reg = xr[wave];
+        C *reg = xr[wave];
         const int lane = tid & 63, li = lane & 15, lg = lane >> 4;
 #pragma unroll
         for (int k = 0; k < kNS; ++k) reg[lane * kXr + k] = x[k] * dval1[k];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-        double av[2][kNS / 4];
+        C av[2][kNS / 4];
 #pragma unroll
         for (int jt = 0; jt < 2; ++jt)
 #pragma unroll
             for (int s = 0; s < kNS / 4; ++s) av[jt][s] = lnext[16 * jt + li][4 * s + lg];
-        v4d acc[2][4];
+        const typename M::acc_t zero = {C(0), C(0), C(0), C(0)};
+        typename M::acc_t acc[2][4];
 #pragma unroll
         for (int rt = 0; rt < 4; ++rt) {
-            acc[0][rt] = v4d{0.0, 0.0, 0.0, 0.0};
-            acc[1][rt] = v4d{0.0, 0.0, 0.0, 0.0};
+            acc[0][rt] = zero;
+            acc[1][rt] = zero;
 #pragma unroll
             for (int s = 0; s < kNS / 4; ++s) {
-                const double bv = reg[(16 * rt + li) * kXr + 4 * s + lg];
-                acc[0][rt] = Mfma16<double>::run(av[0][s], bv, acc[0][rt]);
-                acc[1][rt] = Mfma16<double>::run(av[1][s], bv, acc[1][rt]);
+                const C bv = reg[(16 * rt + li) * kXr + 4 * s + lg];
+                acc[0][rt] = M::run(av[0][s], bv, acc[0][rt]);
+                acc[1][rt] = M::run(av[1][s], bv, acc[1][rt]);
             }
         }
         // lane holds S[16 rt + li][16 jt + row(lane, v)]: back to the row owners through the region
@@ -312,7 +315,7 @@ __device__ __forceinline__ void panel_rows(const BandDesc &d, int c0)
 #pragma unroll
             for (int jt = 0; jt < 2; ++jt)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) reg[(16 * rt + li) * kXr + 16 * jt + Mfma16<double>::row(lane, v)] = acc[jt][rt][v];
+                for (int v = 0; v < 4; ++v) reg[(16 * rt + li) * kXr + 16 * jt + M::row(lane, v)] = acc[jt][rt][v];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 #pragma unroll
@@ -323,10 +326,10 @@ __device__ __forceinline__ void panel_rows(const BandDesc &d, int c0)
         const int rr = r - kNS;
 #pragma unroll
         for (int kk = 0; kk < kNS; ++kk)
-            blk2[kk * kNS + rr] = rr >= w2 ? (rr == kk ? 1.0 : 0.0) : (kk <= rr ? x[kNS + kk] : 0.0);
+            blk2[kk * kNS + rr] = rr >= w2 ? (rr == kk ? C(1) : C(0)) : (kk <= rr ? x[kNS + kk] : C(0));
     }
     __syncthreads();
-    if (wave == 0) diag_factor<T>(blk2, dval2, dinv2, w2, c0 + kNS, d.info, false);
+    if (wave == 0) diag_factor<T, C>(blk2, dval2, dinv2, w2, c0 + kNS, d.info, false);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kQ; ++q) {
@@ -338,7 +341,7 @@ __device__ __forceinline__ void panel_rows(const BandDesc &d, int c0)
     // TRSM of the second sub-panel
 #pragma unroll
     for (int kk = 0; kk < kNS; ++kk) {
-        const double tk = x[kNS + kk];
+        const C tk = x[kNS + kk];
 #pragma unroll
         for (int jj = kk + 1; jj < kNS; ++jj) x[kNS + jj] -= tk * blk2[kk * kNS + jj];
         x[kNS + kk] = tk * dinv2[kk];

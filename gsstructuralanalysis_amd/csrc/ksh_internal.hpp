@@ -198,6 +198,9 @@ struct RefDesc {
     double *x, *r, *norms;
     double *part;          // 2 doubles per 64-row block (partial norms)
     int blk0, nblk;        // set by the launcher: first row block in the launch, row blocks
+    // a second vector on the same matrix (set by the launcher when two jobs share ab; else null)
+    const double *b2;
+    double *x2, *r2, *norms2, *part2;
 };
 int refine_residual_batch(hipStream_t s, RefDesc *d_descs, const std::vector<RefDesc> &h);
 int refine_update_batch(hipStream_t s, RefDesc *d_descs, const std::vector<RefDesc> &h);

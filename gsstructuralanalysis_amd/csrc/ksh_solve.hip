@@ -432,6 +432,14 @@ int band_solve_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHos
         bwmax = std::max(bwmax, h[i].bw);
         bytes_max = std::max(bytes_max, (size_t)h[i].n * (h[i].bw + 1) * es);
     }
+    // Largest bands first: a workgroup's time is about its band's bytes (one CU streams them), and
+    // workgroups are dispatched in index order as CUs free up, so with more bands than CUs
+    // (two right-hand sides per matrix) the long ones start at once and the short ones fill in
+    // behind them (longest-processing-time-first), instead of a second round of long ones.
+    std::stable_sort(dd.begin(), dd.end(), [](const BandDesc &a, const BandDesc &b) {
+        const size_t ca = a.active ? (size_t)a.n * (size_t)(a.bw + 1) : 0, cb = b.active ? (size_t)b.n * (size_t)(b.bw + 1) : 0;
+        return ca > cb;
+    });
     KSH_HIP(hipMemcpyAsync(d_descs, dd.data(), sizeof(BandDesc) * dd.size(), hipMemcpyHostToDevice, s));
     const size_t smem = solve_lds(nmax, bwmax, bytes_max, prec);
     ProfScope ps(prec == KSH_FP64 ? "band_solve_batch_fp64" : "band_solve_batch_fp32", s);
