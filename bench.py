@@ -18,6 +18,9 @@ steps / wall seconds of the headline form, inputs resident on the GPU:
   "mixed+learned"   (default; the metric's form) FP32 band LDL^T with FP64 refinement, FP64 K, R,
                     constraint and pivot count, and the learned arc-length predictor;
   "mixed"           the same with the tangent predictor;
+  "fp32k+learned"   K itself assembled in FP32 (element GEMMs on FP32 MFMA, FP32 band), FP32 LDL^T,
+                    FP64 refinement against the FP32 K, FP64 R, constraint and pivot count (an FP64
+                    re-assembly and factor where the count is not certified), learned predictor;
   "batched+learned" FP64 throughout with the learned predictor;
   "batched"         FP64 throughout with the tangent predictor.
 The headline run is timed with profiling off; a second run of the same form with per-launch HIP
@@ -54,7 +57,8 @@ KERNELS = ["tangent_gp", "tangent_elem", "band_gather", "residual_elem", "vector
            "band_symv", "band_ldlt_batch_fp64", "band_ldlt_batch_fp32", "band_solve_batch_fp64", "band_solve_batch_fp32",
            "refine_residual_batch", "band_ldlt_early_fp64", "fp64_refactor_pivot", "fp64_refactor_stall",
            "fp64_refactor_early", "fp64_refactor_early_used", "fp64_refactor_count", "crisfield_complex_roots",
-           "band_ldlt_tiled", "band_solve_tiled", "engine_rounds", "engine_round_us"]
+           "band_ldlt_tiled", "band_solve_tiled", "band_ldlt_recount_fp64", "fp64_recount_deferred", "engine_rounds",
+           "engine_round_us"]
 C2_STEPS, C1_STEPS = 60, 80
 C2_KAPPA = 2.6
 
@@ -80,14 +84,14 @@ def kernel_stats(lib):
 
 # bound of each kernel: MFMA (work in flop) or HBM (work in bytes)
 BOUND = {"band_ldlt": "mfma", "band_ldlt_batch_fp64": "mfma", "band_ldlt_batch_fp32": "mfma", "band_ldlt_tiled": "mfma",
-         "band_ldlt_early_fp64": "mfma",
+         "band_ldlt_early_fp64": "mfma", "band_ldlt_recount_fp64": "mfma",
          "tangent_elem": "mfma", "tangent_gp": "hbm", "band_solve": "hbm", "band_solve_batch_fp64": "hbm",
          "band_solve_batch_fp32": "hbm", "band_solve_tiled": "hbm", "band_gather": "hbm", "band_symv": "hbm",
          "refine_residual_batch": "hbm"}
 FP32_KERNELS = {"band_ldlt_batch_fp32"}
-# launched on the engine's second stream, beside the main stream's work: reported, but not a
+# launched on the engine's second and third streams, beside the main stream's work: reported, but not a
 # candidate for the dominant kernel (their time overlaps the main stream's)
-OVERLAPPED = {"band_ldlt_early_fp64"}
+OVERLAPPED = {"band_ldlt_early_fp64", "band_ldlt_recount_fp64"}
 
 
 def measured_traffic(kernel):
@@ -230,7 +234,8 @@ def main():
     ap.add_argument("--no-c4", action="store_true", help="skip the C4 eigen-analysis leg")
     ap.add_argument("--plain-sample", type=int, default=2, help="plain FP64: instances timed per size class")
     ap.add_argument("--plain-all", action="store_true", help="plain FP64: time every instance")
-    ap.add_argument("--form", choices=["mixed+learned", "mixed", "batched+learned", "batched"], default="mixed+learned",
+    ap.add_argument("--form", choices=["mixed+learned", "mixed", "fp32k+learned", "batched+learned", "batched"],
+                    default="mixed+learned",
                     help="headline form")
     ap.add_argument("--epochs", type=int, default=400, help="training epochs of the learned predictor")
     ap.add_argument("--only", action="store_true",
@@ -274,7 +279,8 @@ def main():
     mixed = dict(precision=L.FP32_IR, max_refine=args.max_refine, refine_tol=args.refine_tol,
                  refine_accept=args.refine_accept)
     forms = {"batched": dict(precision=L.FP64), "mixed": mixed,
-             "batched+learned": dict(precision=L.FP64), "mixed+learned": dict(mixed)}
+             "batched+learned": dict(precision=L.FP64), "mixed+learned": dict(mixed),
+             "fp32k+learned": dict(mixed, precision=L.FP32)}
 
     t_build = time.perf_counter()
     if args.workload == "c1":
@@ -498,12 +504,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.form.startswith("batched") else "f64 (K, R, refinement, constraint, count) + f32 (band LDL^T)",
+        "dtype": ("f64" if args.form.startswith("batched") else
+                  "f32 (K assembly, band LDL^T) + f64 (R, refinement, constraint, count)" if args.form.startswith("fp32k")
+                  else "f64 (K, R, refinement, constraint, count) + f32 (band LDL^T)"),
         "data": "synthetic",
         "config": {"workload": workload,
                    "form": {"batched": "FP64 + batched", "mixed": "mixed + batched",
                             "batched+learned": "FP64 + batched + learned predictor",
-                            "mixed+learned": "mixed + batched + learned predictor"}[args.form]
+                            "mixed+learned": "mixed + batched + learned predictor",
+                            "fp32k+learned": "FP32 K + FP32 factor + FP64 refinement + batched + learned predictor"}[args.form]
                            + ("" if len(sps) > 1 else ", single instance"),
                    "instances": len(sps), "parallelism": "single GPU"},
         "steps_meaning": (f"arc-length steps of the whole batch (every instance one converged step); "

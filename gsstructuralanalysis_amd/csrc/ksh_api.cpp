@@ -162,7 +162,12 @@ void destroy_matrix(Matrix *K)
         (void)hipEventDestroy(K->early_done);
         K->early_done = nullptr;
     }
-    void *ptrs[] = {K->ab, K->fac, K->fac64, K->u_at, K->work, K->flag, K->red, K->desc};
+    if (K->rc_done) {  // a deferred pivot count of a batch engine may still be factoring rc64
+        (void)hipEventSynchronize(K->rc_done);
+        (void)hipEventDestroy(K->rc_done);
+        K->rc_done = nullptr;
+    }
+    void *ptrs[] = {K->ab, K->fac, K->fac64, K->rc64, K->u_at, K->work, K->flag, K->red, K->desc};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -241,6 +246,13 @@ int ensure_fac64(Matrix &K)
 {
     int rc = 0;
     if (!K.fac64 && !(K.fac64 = dalloc<double>((size_t)K.n * K.ld, rc))) return rc;
+    return KSH_SUCCESS;
+}
+
+int ensure_rc64(Matrix &K)
+{
+    int rc = 0;
+    if (!K.rc64 && !(K.rc64 = dalloc<double>((size_t)K.n * K.ld, rc))) return rc;
     return KSH_SUCCESS;
 }
 

@@ -31,6 +31,15 @@ BatchDevice::~BatchDevice()
         (void)hipStreamSynchronize(m_early_stream);
         (void)hipStreamDestroy(m_early_stream);
     }
+    if (m_rc_stream) {
+        (void)hipStreamSynchronize(m_rc_stream);
+        (void)hipStreamDestroy(m_rc_stream);
+    }
+    for (auto &b : m_rcb) {
+        if (b->done) (void)hipEventDestroy(b->done);
+        if (b->host) (void)hipHostFree(b->host);
+    }
+    if (m_ev_rc_ready) (void)hipEventDestroy(m_ev_rc_ready);
     if (m_stream) (void)hipStreamSynchronize(m_stream);
     if (m_ev_copied) (void)hipEventDestroy(m_ev_copied);
     if (m_ev_early) (void)hipEventDestroy(m_ev_early);
@@ -335,6 +344,131 @@ int BatchDevice::refactor64(const std::vector<Matrix *> &ms)
         prof_count("fp64_refactor_early_used", (long long)taken.size());
     }
     return refactor64_batch(re);
+}
+
+int BatchDevice::recount_async(const std::vector<Matrix *> &ms, const std::vector<long long> &tags, std::vector<Count> &now)
+{
+    if (ms.size() != tags.size()) return fail(KSH_OTHER_ERROR, "recount_async: one tag per matrix");
+    // matrices with an early FP64 factor of their current values: its statistics now
+    std::vector<Matrix *> rest;
+    std::vector<long long> rest_tags;
+    bool early_waited = false;
+    for (size_t i = 0; i < ms.size(); ++i) {
+        const long k = early_index(ms[i]);
+        if (k < 0) {
+            rest.push_back(ms[i]);
+            rest_tags.push_back(tags[i]);
+            continue;
+        }
+        if (!early_waited) {
+            KSH_HIP(hipEventSynchronize(m_ev_early));
+            early_waited = true;
+        }
+        const size_t ne = m_early.size();
+        const int *hi = (const int *)m_hearly;
+        const double *hs = (const double *)((const char *)m_hearly + ne * sizeof(int));
+        Count c;
+        c.tag = tags[i];
+        c.info = hi[k];
+        for (int q = 0; q < 4; ++q) c.stats[q] = hs[4 * k + q];
+        now.push_back(c);
+    }
+    if (!early_waited && rest.empty()) return KSH_SUCCESS;
+    if (early_waited) prof_count("fp64_refactor_early_used", (long long)(ms.size() - rest.size()));
+    if (rest.empty()) return KSH_SUCCESS;
+    int rc;
+    if (!m_rc_stream) {
+        KSH_HIP(hipStreamCreateWithFlags(&m_rc_stream, hipStreamNonBlocking));
+        KSH_HIP(hipEventCreateWithFlags(&m_ev_rc_ready, hipEventDisableTiming));
+    }
+    RecountBatch *B = nullptr;  // a record whose counts were taken, else a new one
+    for (auto &b : m_rcb)
+        if (!b->busy) {
+            B = b.get();
+            break;
+        }
+    if (!B) {
+        m_rcb.push_back(std::make_unique<RecountBatch>());
+        B = m_rcb.back().get();
+        KSH_HIP(hipEventCreateWithFlags(&B->done, hipEventDisableTiming));
+    }
+    const size_t n = rest.size();
+    const size_t hb = n * (sizeof(int) + 4 * sizeof(double));
+    if ((rc = ensure_desc(n)) || (rc = B->desc.ensure(n * kBandDescBytes)) || (rc = B->info.ensure(n * sizeof(int))) ||
+        (rc = B->stats.ensure(n * 4 * sizeof(double))))
+        return rc;
+    if (B->host_bytes < hb) {
+        if (B->host) (void)hipHostFree(B->host);
+        B->host = nullptr;
+        B->host_bytes = 0;
+        KSH_HIP(hipHostMalloc(&B->host, hb, hipHostMallocDefault));
+        B->host_bytes = hb;
+    }
+    std::vector<CopyDesc> cp;
+    std::vector<InstDesc> tang;
+    std::vector<BandDescHost> h;
+    for (size_t k = 0; k < n; ++k) {
+        Matrix *K = rest[k];
+        if ((rc = ensure_rc64(*K))) return rc;
+        if (K->rc_done) KSH_HIP(hipStreamWaitEvent(m_stream, K->rc_done, 0));  // rc64 free again
+        if (K->sprec == KSH_FP32 && K->has_u) {
+            InstDesc d = make_desc(*K->pb);
+            d.u_free = K->u_at;
+            d.ab = K->rc64;
+            d.sprec = KSH_FP64;
+            tang.push_back(d);
+        } else {
+            cp.push_back(CopyDesc{K->ab, K->rc64, (size_t)K->n * K->ld, K->sprec, KSH_FP64, 0});
+        }
+        h.push_back(BandDescHost{K->rc64, K->n, K->bw, (int *)B->info.p + k, (double *)B->stats.p + 4 * k, nullptr, 1});
+    }
+    if ((rc = band_copy_batch(m_stream, m_cdesc.p, cp))) return rc;
+    if (!tang.empty() && (rc = batch_tangent(m_stream, (InstDesc *)m_desc.p, tang))) return rc;
+    KSH_HIP(hipEventRecord(m_ev_rc_ready, m_stream));
+    KSH_HIP(hipStreamWaitEvent(m_rc_stream, m_ev_rc_ready, 0));
+    if ((rc = band_factor_batch(m_rc_stream, B->desc.p, h, KSH_FP64, "band_ldlt_recount_fp64"))) {
+        (void)hipStreamSynchronize(m_rc_stream);  // nothing of a failed launch left in flight
+        return rc;
+    }
+    char *hp = (char *)B->host;
+    KSH_HIP(hipMemcpyAsync(hp, B->info.p, n * sizeof(int), hipMemcpyDeviceToHost, m_rc_stream));
+    KSH_HIP(hipMemcpyAsync(hp + n * sizeof(int), B->stats.p, n * 4 * sizeof(double), hipMemcpyDeviceToHost, m_rc_stream));
+    KSH_HIP(hipEventRecord(B->done, m_rc_stream));
+    for (Matrix *K : rest) {
+        if (!K->rc_done) KSH_HIP(hipEventCreateWithFlags(&K->rc_done, hipEventDisableTiming));
+        KSH_HIP(hipEventRecord(K->rc_done, m_rc_stream));
+    }
+    B->tags = rest_tags;
+    B->n = n;
+    B->busy = true;
+    m_refactors += (long long)n;
+    prof_count("fp64_recount_deferred", (long long)n);
+    return KSH_SUCCESS;
+}
+
+int BatchDevice::recount_take(bool wait, std::vector<Count> &out)
+{
+    for (auto &b : m_rcb) {
+        if (!b->busy) continue;
+        if (wait) {
+            KSH_HIP(hipEventSynchronize(b->done));
+        } else {
+            const hipError_t e = hipEventQuery(b->done);
+            if (e == hipErrorNotReady) continue;
+            if (e != hipSuccess) return fail(KSH_OTHER_ERROR, std::string("deferred pivot count: ") + hipGetErrorString(e));
+        }
+        const int *hi = (const int *)b->host;
+        const double *hs = (const double *)((const char *)b->host + b->n * sizeof(int));
+        for (size_t k = 0; k < b->n; ++k) {
+            Count c;
+            c.tag = b->tags[k];
+            c.info = hi[k];
+            for (int q = 0; q < 4; ++q) c.stats[q] = hs[4 * k + q];
+            out.push_back(c);
+        }
+        b->busy = false;
+    }
+    return KSH_SUCCESS;
 }
 
 // Early FP64 factors: a matrix whose previous FP32 factor needed the FP64 fallback in a solve is

@@ -6,6 +6,7 @@
 // device memory; only per-instance scalars (pivot statistics, norms) come back to the host.
 #pragma once
 
+#include <memory>
 #include <vector>
 
 #include "ksh_internal.hpp"
@@ -77,6 +78,22 @@ public:
     // steps z <- z - M^-1 K z from the fixed pseudo-random start
     int power_contraction(const std::vector<Matrix *> &ms);
 
+    // Deferred FP64 pivot counts, for counts that are only recorded (the path does not branch on
+    // them).  Each matrix's current values are copied (FP32 storage: re-assembled in FP64 at the
+    // stored point) on the engine stream into a buffer of the matrix's own, factored in FP64 on a
+    // third stream of default priority, and their statistics copied to pinned host memory, while
+    // the engine stream goes on with the next round.  A matrix with an early FP64 factor of its
+    // current values takes that factor's statistics at once instead (appended to `now`).
+    // recount_take returns the deferred counts whose stream work has completed (all of them with
+    // wait), each with the caller's tag.
+    struct Count {
+        long long tag;
+        int info;          // 0, or the factor hit a zero / non-finite pivot
+        double stats[4];   // negatives, min d, max |d|, min |d|
+    };
+    int recount_async(const std::vector<Matrix *> &ms, const std::vector<long long> &tags, std::vector<Count> &now);
+    int recount_take(bool wait, std::vector<Count> &out);
+
     int dots(const std::vector<DotDesc> &d, std::vector<double> &host_out);  // npairs doubles each, in order
     int vecops(const std::vector<VecDesc> &d);
     int synchronize();
@@ -103,6 +120,17 @@ private:
     DevScratch m_ebdesc, m_einfo, m_estats;
     void *m_hearly = nullptr;  // pinned: info + stats of the early factors
     size_t m_hearly_bytes = 0;
+    struct RecountBatch {
+        hipEvent_t done = nullptr;
+        DevScratch desc, info, stats;
+        void *host = nullptr;  // pinned: info + stats
+        size_t host_bytes = 0, n = 0;
+        std::vector<long long> tags;
+        bool busy = false;
+    };
+    std::vector<std::unique_ptr<RecountBatch>> m_rcb;
+    hipStream_t m_rc_stream = nullptr;
+    hipEvent_t m_ev_rc_ready = nullptr;
     DevScratch m_desc, m_bdesc, m_cdesc, m_rdesc, m_stats, m_info, m_norms, m_part, m_r, m_dotdesc, m_vecdesc, m_dotout;
     DevScratch m_zero, m_ppart, m_pw, m_start;
     size_t m_start_n = 0;

@@ -16,7 +16,10 @@
 //      each panel row below it is solved against L_dd D (row-wise TRSM); between the two
 //      sub-panels the second one's columns take the first one's rank-32 update (on MFMA in the
 //      register-resident form).  (One workgroup per band: a second one would read the diagonal
-//      block while the first writes it back.)
+//      block while the first writes it back.)  The register-resident form (bw <= 480) computes in
+//      the storage precision: in FP32 its 101 VGPRs and 80 KB of LDS let a panel workgroup share
+//      its CU with a trailing-update workgroup of the same step launch (in FP64, 212 VGPRs and
+//      157 KB, one workgroup per CU); the general form (wider bands) computes in FP64.
 //   2. trailing update: the lower triangle of the bw x bw trailing window is cut into 32 x 32
 //      blocks, one per wave, each a 2 x 2 set of 16 x 16 MFMA tiles (v_mfma_f64_16x16x4f64 /
 //      v_mfma_f32_16x16x4f32) over the NB panel columns; panel operands -d_k L[j][k] and

@@ -136,6 +136,8 @@ struct Matrix {
     bool ab_clean = false;         // every band slot that no element reaches holds 0 in ab
     bool stalled = false;          // a solve of the current FP32 factor fell back to FP64 (batch engine)
     hipEvent_t early_done = nullptr;  // recorded after the last early FP64 factor into fac64 (batch engine)
+    double *rc64 = nullptr;        // FP64 copy factored for a deferred pivot count (batch engine)
+    hipEvent_t rc_done = nullptr;  // recorded after the last deferred count's factor of rc64
     uint64_t version = 0;          // bumped on every change of ab
     uint64_t factored_version = ~0ull;
     int info = 0;                  // 0 or 1-based index of a zero / non-finite pivot
@@ -208,6 +210,8 @@ int refine_update_batch(hipStream_t s, RefDesc *d_descs, const std::vector<RefDe
 int refactor_fp64(Matrix &K);
 // allocates K.fac64 if it is not there yet
 int ensure_fac64(Matrix &K);
+// allocates K.rc64 (the deferred pivot count's FP64 copy) if it is not there yet
+int ensure_rc64(Matrix &K);
 int band_solve_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHost> &h, int prec);
 constexpr size_t kBandDescBytes = 48;
 int band_solve(Matrix &K, const void *fac, int prec, double *x_dev);

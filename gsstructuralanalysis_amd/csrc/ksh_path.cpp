@@ -141,6 +141,7 @@ public:
         if ((rc = setup(pbs, count, res))) return rc;
         if ((rc = initialize())) return rc;
         for (;;) {
+            if ((rc = take_counts(false))) return rc;
             std::vector<Inst *> live;
             for (Inst &I : m_in)
                 if (I.ph != Phase::Done) live.push_back(&I);
@@ -150,6 +151,7 @@ public:
             prof_count("engine_rounds", 1);
             prof_count("engine_round_us", (long long)((now_s() - r0) * 1e6));
         }
+        if ((rc = take_counts(true))) return rc;
         if ((rc = m_dev.synchronize())) return rc;
         const double t1 = now_s();
         if (wall) *wall = t1 - t0;
@@ -833,16 +835,32 @@ private:
             if (std::max(K->rho, K->rho_pw) > K->guard) recount.push_back(K);
         if (!recount.empty()) {
             prof_count("fp64_refactor_count", (long long)recount.size());
-            if ((rc = m_dev.refactor64(recount))) return rc;
-            for (Inst *I : conv)
-                if (std::find(recount.begin(), recount.end(), I->K) != recount.end()) {
-                    if (I->K->info) {
-                        I->status = fail(KSH_SOLVER_ERROR, "FP64 refactorization for the pivot count failed");
-                        I->ph = Phase::Done;
-                        continue;
+            if (!m_o.singular_point) {
+                // the count is only recorded (no singular-point procedure branches on it): its FP64
+                // factor runs beside the next rounds, and the record is completed when it is done
+                std::vector<long long> tags;
+                for (Matrix *K : recount)
+                    for (Inst *I : conv)
+                        if (I->K == K) {
+                            tags.push_back((long long)m_pending.size());
+                            m_pending.push_back(PendingCount{I, I->k});
+                            break;
+                        }
+                std::vector<BatchDevice::Count> now;
+                if ((rc = m_dev.recount_async(recount, tags, now))) return rc;
+                for (const BatchDevice::Count &c : now) apply_count(c, true);
+            } else {
+                if ((rc = m_dev.refactor64(recount))) return rc;
+                for (Inst *I : conv)
+                    if (std::find(recount.begin(), recount.end(), I->K) != recount.end()) {
+                        if (I->K->info) {
+                            I->status = fail(KSH_SOLVER_ERROR, "FP64 refactorization for the pivot count failed");
+                            I->ph = Phase::Done;
+                            continue;
+                        }
+                        count_pivots(*I);
                     }
-                    count_pivots(*I);
-                }
+            }
         }
         std::vector<VecDesc> vd;
         std::vector<Inst *> fin;
@@ -911,6 +929,43 @@ private:
             rec.push_back(I);
         }
         return record_steps(rec);
+    }
+
+    // A deferred FP64 pivot count of step p.k: before that step is recorded (an early factor's
+    // statistics), it replaces the instance's count as the synchronous count would; afterwards it
+    // replaces the recorded count.  A count whose factor failed ends the path at that step, where
+    // the synchronous count ends it.
+    void apply_count(const BatchDevice::Count &c, bool before_record)
+    {
+        const PendingCount &p = m_pending[(size_t)c.tag];
+        Inst &I = *p.I;
+        if (c.info) {
+            if (before_record || p.k < I.k) {
+                I.status = fail(KSH_SOLVER_ERROR, "FP64 refactorization for the pivot count failed");
+                I.k = std::min(I.k, p.k);
+                I.ph = Phase::Done;
+            }
+            return;
+        }
+        if (before_record) {
+            I.negatives = (int64_t)c.stats[0];
+            I.indicator = c.stats[1];
+            return;
+        }
+        if (p.k >= I.k) return;  // the path was ended before this step
+        ksh_alm_result &r = *I.res;
+        if (r.negatives) r.negatives[p.k] = (int64_t)c.stats[0];
+        if (r.indicator) r.indicator[p.k] = c.stats[1];
+    }
+
+    int take_counts(bool wait)
+    {
+        if (m_pending.empty()) return KSH_SUCCESS;
+        std::vector<BatchDevice::Count> done;
+        int rc;
+        if ((rc = m_dev.recount_take(wait, done))) return rc;
+        for (const BatchDevice::Count &c : done) apply_count(c, false);
+        return KSH_SUCCESS;
     }
 
     // the driver's record of converged points (Uold, Lold) and its arc-length reset
@@ -1415,6 +1470,11 @@ private:
     ksh_alm_options m_o;
     BatchDevice m_dev;
     std::vector<Inst> m_in;
+    struct PendingCount {  // a deferred pivot count (the tag is its index)
+        Inst *I;
+        int k;             // the step it belongs to
+    };
+    std::vector<PendingCount> m_pending;
     Slab m_slab;
     MatrixOwner m_mats;
     double *m_coef = nullptr;
