@@ -205,6 +205,39 @@ __global__ __launch_bounds__(64 * E) void k_tangent_tiles(const InstDesc *descs,
     typename M::acc_t acc[6];
 #pragma unroll
     for (int t = 0; t < 6; ++t) acc[t] = typename M::acc_t{0, 0, 0, 0};
+    if constexpr (NG % 4 == 0) {
+        // inner index kk in steps of one component a (uniform over the wave: every H_g offset
+        // below is an immediate) at the point g = 4 gi + kq of the lane's group, whose basis
+        // values are formed once per gi instead of once per step
+#pragma unroll 1
+        for (int gi = 0; gi < NG / 4; ++gi) {
+            const int g = 4 * gi + kq, qx = g % NQ, qy = g / NQ;
+            const double *tx = d.tabx + ((size_t)(ex * NQ + qx) * 3) * P1;
+            const double *ty = d.taby + ((size_t)(ey * NQ + qy) * 3) * P1;
+            double psi[5];
+            psi[0] = inA ? tx[P1 + ia] * ty[ib] : 0.0;
+            psi[1] = inA ? tx[ia] * ty[P1 + ib] : 0.0;
+            psi[2] = inA ? tx[2 * P1 + ia] * ty[ib] : 0.0;
+            psi[3] = inA ? tx[ia] * ty[2 * P1 + ib] : 0.0;
+            psi[4] = inA ? tx[P1 + ia] * ty[P1 + ib] : 0.0;
+            const double *Hg = sHw[g];
+#pragma unroll
+            for (int a = 0; a < 5; ++a) {
+                const T aop = (T)psi[a];
+                int t = 0;
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int k = i; k < 3; ++k, ++t) {
+                        double z = 0.0;
+#pragma unroll
+                        for (int b = 0; b < 5; ++b) z += Hg[hq_index(3 * a + i, 3 * b + k)] * psi[b];
+                        acc[t] = M::run(aop, (T)z, acc[t]);
+                    }
+                __builtin_amdgcn_sched_barrier(0);  // one component's H_g reads in flight at a time
+            }
+        }
+    } else
     for (int s = 0; s < NKK / 4; ++s) {
         const int kk = 4 * s + kq;
         const bool ok = inA && kk < 5 * NG;
