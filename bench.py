@@ -233,6 +233,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 reference timing")
     ap.add_argument("--no-c4", action="store_true", help="skip the C4 eigen-analysis leg")
+    ap.add_argument("--c4-only", action="store_true", help="run the C4 eigen-analysis leg alone and print its JSON")
     ap.add_argument("--plain-sample", type=int, default=2, help="plain FP64: instances timed per size class")
     ap.add_argument("--plain-all", action="store_true", help="plain FP64: time every instance")
     ap.add_argument("--form", choices=["mixed+learned", "mixed", "fp32k+learned", "batched+learned", "batched"],
@@ -255,6 +256,10 @@ def main():
                     help="only copy a MIB-MiB device buffer 5 times (device-to-device), for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
     args = ap.parse_args()
+
+    if args.c4_only:
+        print(json.dumps(c4_leg()), flush=True)
+        return
 
     if args.copy_calibration:
         import torch
