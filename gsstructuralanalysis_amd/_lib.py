@@ -115,6 +115,7 @@ SIGNATURES = {
     "ksh_problem_create": (C.c_int, [C.POINTER(Patch), C.POINTER(Material), C.POINTER(BC), C.POINTER(Loads),
                                      C.POINTER(_VP)]),
     "ksh_problem_destroy": (C.c_int, [_VP]),
+    "ksh_problem_set_pressure": (C.c_int, [_VP, C.c_double]),
     "ksh_num_free_dofs": (C.c_int, [_VP, _i64p]),
     "ksh_bandwidth": (C.c_int, [_VP, _i64p]),
     "ksh_synchronize": (C.c_int, [_VP]),

@@ -147,7 +147,7 @@ void destroy_problem(Problem *pb)
 {
     if (!pb) return;
     void *ptrs[] = {pb->ctrl0, pb->free_idx, pb->free2full, pb->tabx, pb->taby, pb->wq, pb->gpgeo,
-                    pb->u_full, pb->u_free, pb->r_free, pb->Ke, pb->Hg, pb->Fe, pb->F, pb->pl_N, pb->pl_span, pb->pl_F,
+                    pb->u_full, pb->u_free, pb->r_free, pb->Ke, pb->Hg, pb->Fe, pb->F, pb->P0, pb->pl_N, pb->pl_span, pb->pl_F,
                     pb->d_desc, pb->modal_P, pb->modal_Phi, pb->modal_zfree};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -414,6 +414,19 @@ int ksh_problem_create(const ksh_patch *patch, const ksh_material *mat, const ks
     if ((rc = launch_external_force(P))) return bail(rc);
     if (hipStreamSynchronize(P.stream) != hipSuccess) return bail(fail(KSH_OTHER_ERROR, "set-up kernels failed"));
     *out = pb;
+    return KSH_SUCCESS;
+}
+
+int ksh_problem_set_pressure(ksh_problem *pb, double pressure)
+{
+    if (!pb) return fail(KSH_OTHER_ERROR, "null argument");
+    if (!std::isfinite(pressure)) return fail(KSH_OTHER_ERROR, "pressure must be finite");
+    int rc = 0;
+    if (pressure != 0.0 && !pb->P0 && !(pb->P0 = dalloc<double>((size_t)pb->nfree, rc))) return rc;
+    pb->pressure = pressure;
+    if ((rc = launch_external_force(*pb))) return rc;                       // the dead loads
+    if (pressure != 0.0 && (rc = launch_pressure_rest(*pb))) return rc;   // P0, and F += P0
+    KSH_HIP(hipStreamSynchronize(pb->stream));
     return KSH_SUCCESS;
 }
 

@@ -21,6 +21,7 @@
 // thread to (instance, dof / band slot) by binary search over the prefix offsets.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "ksh_devutil.hpp"
@@ -104,7 +105,14 @@ __global__ __launch_bounds__(128) void k_residual_gp(const InstDesc *descs, int 
     shell::gp_state(dv, d.gpgeo + (size_t)gp * kGeoStride, d.t, st);
     const size_t stride = (size_t)d.nel * NG;
 #pragma unroll
-    for (int I = 0; I < 15; ++I) d.Hg[(size_t)I * stride + gp] = shell::f_entry(st, I);
+    for (int I = 0; I < kFq; ++I) d.Hg[(size_t)I * stride + gp] = shell::f_entry(st, I);
+    if (d.pressure != 0.0) {  // -p w (a1 x a2) on the deformed surface: the follower load, as a force on the residual's side
+        double n[3];
+        shell::cross3(dv, dv + 3, n);
+        const double s = -d.pressure * d.gpgeo[(size_t)gp * kGeoStride + 13];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) d.Hg[(size_t)(kFq + i) * stride + gp] = s * n[i];
+    }
 }
 
 // Residual, stage 2: one wave per element; lane (A, i) sums Fe[3A+i] = sum_g sum_a
@@ -131,6 +139,7 @@ __global__ __launch_bounds__(256) void k_residual_tiles(const InstDesc *descs, i
                                tx[ia] * ty[2 * P1 + ib], tx[P1 + ia] * ty[P1 + ib]};
 #pragma unroll
         for (int a = 0; a < 5; ++a) acc += psi[a] * Fg[(size_t)(3 * a + i) * stride + g];
+        if (d.pressure != 0.0) acc += tx[ia] * ty[ib] * Fg[(size_t)(kFq + i) * stride + g];
     }
     d.Fe[(size_t)e * 3 * NB + lane] = acc;
 }
@@ -285,6 +294,73 @@ __global__ __launch_bounds__(64 * E) void k_tangent_tiles(const InstDesc *descs,
             }
 }
 
+// (e_k x a)_i
+__device__ __forceinline__ double ek_cross(int k, const double *a, int i)
+{
+    if (i == k) return 0.0;
+    const int j = 3 - i - k;
+    return ((i - k + 3) % 3 == 1) ? -a[j] : a[j];
+}
+
+// Follower-pressure part of the element tangent, K_e[r][c] -= dP_r / du_c with
+// dP_{A,i} / du_{B,k} = p sum_g w_g N_A (N_B,1 (e_k x a2) + N_B,2 (a1 x e_k))_i (a1, a2 deformed),
+// on the lower triangle r >= c that the band gather reads: the follower tangent is not
+// symmetric, and its lower triangle is what a lower-triangle LDL^T of the reference's assembled
+// matrix reads.  One wave per element, lane r = 3A + i; the element's Gauss-point data (a1, a2,
+// p w, N, N,1, N,2) staged in LDS.
+template <int P, typename T>
+__global__ __launch_bounds__(64) void k_pressure_tiles(const InstDesc *descs, int ninst, int total_el)
+{
+    constexpr int P1 = P + 1, NB = P1 * P1, NQ = P1, NG = NQ * NQ;
+    const int eg = (int)blockIdx.x;
+    if (eg >= total_el) return;
+    const InstDesc &d = descs[find_inst(ninst, eg, [&](int m) { return (long long)descs[m].elem0; })];
+    if (d.pressure == 0.0) return;
+    const int e = eg - d.elem0, ex = e % d.nx, ey = e / d.nx;
+    const int lane = (int)threadIdx.x;
+    __shared__ double sN[NG][3][NB];  // N, N_,1, N_,2 per point
+    __shared__ double sA[NG][7];      // a1, a2, p w per point
+    for (int idx = lane; idx < NG * NB; idx += 64) {
+        const int g = idx / NB, A = idx % NB, qx = g % NQ, qy = g / NQ, ia = A % P1, ib = A / P1;
+        const double *tx = d.tabx + ((size_t)(ex * NQ + qx) * 3) * P1;
+        const double *ty = d.taby + ((size_t)(ey * NQ + qy) * 3) * P1;
+        sN[g][0][A] = tx[ia] * ty[ib];
+        sN[g][1][A] = tx[P1 + ia] * ty[ib];
+        sN[g][2][A] = tx[ia] * ty[P1 + ib];
+    }
+    __syncthreads();
+    if (lane < NG) {
+        const int g = lane;
+        double a1[3] = {0.0, 0.0, 0.0}, a2[3] = {0.0, 0.0, 0.0};
+        for (int A = 0; A < NB; ++A) {
+            const int cp = (ex + A % P1) + d.mx * (ey + A / P1);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double x = d.ctrl0[3 * cp + c] + d.u_full[3 * cp + c];
+                a1[c] += sN[g][1][A] * x;
+                a2[c] += sN[g][2][A] * x;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            sA[g][c] = a1[c];
+            sA[g][3 + c] = a2[c];
+        }
+        sA[g][6] = d.pressure * d.gpgeo[((size_t)e * NG + g) * kGeoStride + 13];
+    }
+    __syncthreads();
+    if (lane >= 3 * NB) return;
+    const int A = lane / 3, i = lane % 3, r = lane;
+    T *Kout = (T *)d.Ke + (size_t)e * kKeTile;
+    for (int c = 0; c <= r; ++c) {
+        const int B = c / 3, k = c % 3;
+        double acc = 0.0;
+        for (int g = 0; g < NG; ++g)
+            acc += sA[g][6] * sN[g][0][A] * (sN[g][1][B] * ek_cross(k, sA[g] + 3, i) - sN[g][2][B] * ek_cross(k, sA[g], i));
+        Kout[c * kKeLd + r] = (T)((double)Kout[c * kKeLd + r] - acc);
+    }
+}
+
 // consistent mass tiles: M_e^{ii}[A][B] = sum_g rho t N_A N_B wdA
 template <int P, typename T>
 __global__ __launch_bounds__(256) void k_mass_elem(const InstDesc *descs, int ninst)
@@ -382,11 +458,15 @@ __global__ void k_setup_geometry(ElemArgs ar, const double *wq, double E, double
     o[10] = shell::dot3(dd[2], a3);
     o[11] = shell::dot3(dd[3], a3);
     o[12] = shell::dot3(dd[4], a3);
-    o[13] = o[14] = o[15] = 0.0;
+    o[13] = wq[g];  // the Gauss weight on the parameter domain (follower pressure)
+    o[14] = o[15] = 0.0;
 }
 
 // Deterministic gather of element vectors into free dofs (fixed element order ey, ex).
-// mode 0: R = sum ; mode 1 (ALM): R = sum - lambda F ; mode 2 (static): R = lambda F - sum
+// mode 0: R = sum ; mode 1 (ALM): R = sum - lambda F ; mode 2 (static): R = lambda F - sum ;
+// mode 3: R = -sum.  With a follower pressure the element sums hold F_int(u) - P(u), and the ALM
+// form adds P(0) (r = F_int - lambda F - (P(u) - P(0)), the reference's Force - lambda Force -
+// rhs(u) with Force = F_ext(0)); the static form is its negative.
 __global__ void k_vector_gather(const InstDesc *descs, int ninst, int total)
 {
     int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -402,8 +482,10 @@ __global__ void k_vector_gather(const InstDesc *descs, int ninst, int total)
             int e = ex + d.nx * ey, A = (i - ex) + P1 * (j - ey);
             acc += d.Fe[(size_t)e * 3 * NB + 3 * A + c];
         }
-    if (d.mode == 1) acc = acc - d.lambda * d.F[I];
-    else if (d.mode == 2) acc = d.lambda * d.F[I] - acc;
+    const double p0 = d.P0 ? d.P0[I] : 0.0;
+    if (d.mode == 1) acc = acc - d.lambda * d.F[I] + p0;
+    else if (d.mode == 2) acc = d.lambda * d.F[I] - acc - p0;
+    else if (d.mode == 3) acc = -acc;
     d.R[I] = acc;
 }
 
@@ -630,6 +712,13 @@ int run_tangent(hipStream_t s, InstDesc *d_desc, std::vector<InstDesc> &h, bool 
         };
         tune_launch("tangent_tiles", tune_bucket((double)t.nel) * 4 + (p2 ? 0 : 1) * 2 + (f64 ? 1 : 0), 3, 2, s,
                     launch_tiles);
+        if (std::any_of(h.begin(), h.end(), [](const InstDesc &d) { return d.pressure != 0.0; })) {
+            const dim3 gp((unsigned)t.nel), bp(64);
+            if (p2 && f64) hipLaunchKernelGGL((k_pressure_tiles<2, double>), gp, bp, 0, s, d_desc, n, tn);
+            else if (p2) hipLaunchKernelGGL((k_pressure_tiles<2, float>), gp, bp, 0, s, d_desc, n, tn);
+            else if (f64) hipLaunchKernelGGL((k_pressure_tiles<3, double>), gp, bp, 0, s, d_desc, n, tn);
+            else hipLaunchKernelGGL((k_pressure_tiles<3, float>), gp, bp, 0, s, d_desc, n, tn);
+        }
     }
     KSH_HIP(hipGetLastError());
     ProfScope ps(mass ? "mass_gather" : "band_gather", s);
@@ -690,6 +779,8 @@ InstDesc make_desc(const Problem &pb)
     d.Fe = pb.Fe;
     d.Ke = pb.Ke;
     d.Hg = pb.Hg;
+    d.pressure = pb.pressure;
+    d.P0 = pb.pressure != 0.0 ? pb.P0 : nullptr;
     return d;
 }
 
@@ -752,7 +843,8 @@ int launch_residual(Problem &pb, double lambda, int conv, double *R_dev)
     return run_residual(pb.stream, pb.d_desc, h, false);
 }
 
-// residual with caller-chosen partial-result storage (u_full: ndof, Hg: 15 ngp, Fe: 3 nb nel doubles)
+// residual with caller-chosen partial-result storage (u_full: ndof, Hg: kFq ngp, or kFqP ngp with a follower
+// pressure, Fe: 3 nb nel doubles)
 int run_residual_into(Problem &pb, const double *u_free_dev, double lambda, int conv, double *R_dev, double *u_full,
                       double *Hg, double *Fe)
 {
@@ -766,6 +858,32 @@ int run_residual_into(Problem &pb, const double *u_free_dev, double lambda, int 
     d.R = R_dev;
     std::vector<InstDesc> h = {d};
     return run_residual(pb.stream, pb.d_desc, h, true);
+}
+
+// y += x
+__global__ void k_add(int n, const double *x, double *y)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += x[i];
+}
+
+// P(0), the follower pressure's load on the undeformed surface, into pb.P0 (the residual kernels at
+// u = 0, where F_int = 0, gathered as -(F_int - P)), then F += P0: the reference's Force =
+// assemble() rhs at zero displacement holds it (examples/gsThinShell_ArcLength.cpp:963-965)
+int launch_pressure_rest(Problem &pb)
+{
+    KSH_HIP(hipMemsetAsync(pb.u_free, 0, sizeof(double) * pb.nfree, pb.stream));
+    InstDesc d = make_desc(pb);
+    d.u_free = pb.u_free;
+    d.mode = 3;
+    d.R = pb.P0;
+    d.P0 = nullptr;
+    std::vector<InstDesc> h = {d};
+    int rc;
+    if ((rc = run_residual(pb.stream, pb.d_desc, h, true))) return rc;
+    hipLaunchKernelGGL(k_add, dim3(nblk(pb.nfree, 256)), dim3(256), 0, pb.stream, pb.nfree, (const double *)pb.P0, pb.F);
+    KSH_HIP(hipGetLastError());
+    return KSH_SUCCESS;
 }
 
 int launch_external_force(Problem &pb)

@@ -60,7 +60,11 @@ struct InstDesc {
     double *u_full, *Fe, *R;
     void *Ke, *ab;
     double *Hg;             // per-Gauss-point H_g (upper triangle, kHq entries), entry-major
+    double pressure;        // follower pressure along a1 x a2 (0: none)
+    const double *P0;       // its load on the undeformed surface (free dofs; with a pressure)
 };
+// entries of a Gauss point's residual record: f_g (15), then -p w (a1 x a2) with a pressure (3)
+constexpr int kFq = 15, kFqP = 18;
 
 struct Problem;
 struct Matrix;
@@ -77,6 +81,8 @@ struct Problem {
     double t = 0, E = 0, nu = 0, rho = 0;
     int npoint = 0;
     double fsurf[3] = {0, 0, 0};
+    double pressure = 0.0;         // follower pressure along a1 x a2 (ksh_problem_set_pressure)
+    double *P0 = nullptr;          // its load on the undeformed surface, free dofs (device)
     int device = 0;
     hipStream_t stream = nullptr;
     double inertia_guard = 0.5;    // pivot-count certificate threshold given to new matrices
@@ -159,6 +165,7 @@ struct Matrix {
 // kernels / launchers (ksh_assembly.hip, ksh_band.hip)
 int launch_setup_geometry(Problem &pb);
 int launch_external_force(Problem &pb);
+int launch_pressure_rest(Problem &pb);  // pb.P0 = P(0) of the follower pressure, F += P0
 int launch_construct_solution(Problem &pb, const double *u_free_dev);
 int launch_residual(Problem &pb, double lambda, int conv, double *R_dev);
 int launch_tangent(Problem &pb, Matrix &K);

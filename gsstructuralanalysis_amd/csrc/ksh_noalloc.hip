@@ -53,11 +53,14 @@ int scratch_ptr(int device, double **out)
 
 }  // namespace
 
-// scratch doubles one residual evaluation of `pb` needs: u_full, 15 resultants per Gauss point,
+// resultants per Gauss point of one residual evaluation (three more with a follower pressure)
+size_t noalloc_fq(const Problem &pb) { return pb.pressure != 0.0 ? (size_t)kFqP : (size_t)kFq; }
+
+// scratch doubles one residual evaluation of `pb` needs: u_full, the resultants per Gauss point,
 // element force vectors
 size_t noalloc_residual_doubles(const Problem &pb)
 {
-    return (size_t)pb.ndof + 15ull * (size_t)pb.ngp + 3ull * (size_t)pb.nb * (size_t)pb.nel + 3 * 32;
+    return (size_t)pb.ndof + noalloc_fq(pb) * (size_t)pb.ngp + 3ull * (size_t)pb.nb * (size_t)pb.nel + 3 * 32;
 }
 
 }  // namespace ksh
@@ -80,7 +83,7 @@ extern "C" int ksh_assemble_residual_dev_noalloc(ksh_problem *pb, const double *
     auto align = [](size_t n) { return (n + 31) & ~size_t(31); };
     double *u_full = base;
     double *Hg = u_full + align((size_t)pb->ndof);
-    double *Fe = Hg + align(15ull * (size_t)pb->ngp);
+    double *Fe = Hg + align(noalloc_fq(*pb) * (size_t)pb->ngp);
     if ((rc = run_residual_into(*pb, u_free_dev, lambda, conv, R_out_dev, u_full, Hg, Fe))) return fail(KSH_ASSEMBLY_ERROR, ksh_last_error());
     return KSH_SUCCESS;
 }

@@ -95,6 +95,7 @@ class Patch:
     point_xi: np.ndarray = field(default_factory=lambda: np.zeros((0, 2)))
     point_F: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
     fsurf: tuple = (0.0, 0.0, 0.0)
+    pressure: float = 0.0                 # follower pressure along a1 x a2 (gsThinShellAssembler::setPressure)
 
     @property
     def mx(self) -> int:

@@ -212,6 +212,14 @@ class ShellProblem:
         L.check(lib.ksh_num_free_dofs(self.handle, C.byref(n)))
         L.check(lib.ksh_bandwidth(self.handle, C.byref(bw)))
         self.nfree, self.bw = n.value, bw.value
+        if getattr(patch, "pressure", 0.0) != 0.0:
+            self.set_pressure(patch.pressure)
+
+    def set_pressure(self, pressure: float):
+        """Follower pressure along a1 x a2 (gsThinShellAssembler::setPressure): F then holds its load
+        on the undeformed surface, the residuals and the tangent its follower terms; 0 removes it."""
+        L.check(L.load().ksh_problem_set_pressure(self.handle, float(pressure)))
+        self.patch.pressure = float(pressure)
 
     def close(self):
         if self.handle:

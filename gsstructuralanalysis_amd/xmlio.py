@@ -5,15 +5,16 @@ Input mirrors the reading section of the reference's XML drivers
 filedata/pde/): the geometry (ID 0: a MultiPatch naming one TensorBSpline2, or the geometry
 itself), the material (ID 10: a MaterialMatrix of type Linear / Linear3, the St. Venant-Kirchhoff
 law; or the older Function IDs 10-13 for thickness, E, nu, density), boundary conditions (ID 20),
-the surface load (ID 21), point loads (IDs 30-32), reference points and values (IDs 50-52) and the
-option lists (IDs 90-95).  The geometry is degree-elevated and uniformly refined as the drivers'
+the surface load (ID 21), the follower pressure (ID 22), point loads (IDs 30-32), reference points
+and values (IDs 50-52) and the option lists (IDs 90-95).  The geometry is degree-elevated and uniformly refined as the drivers'
 -e / -r switches do, and handed to the problem builder as an interpolating control net of the
 refined space (which contains the given surface, so the surface is reproduced exactly).
 
 What the GPU problem class cannot represent raises XmlError naming the item: several patches,
 rational (NURBS) geometry, non-uniform or repeated interior knots together with degree elevation,
 unequal degrees in the two directions, other material laws, non-zero Dirichlet values,
-"Collapsed" / weak conditions, non-constant functions, and follower pressure (ID 22 != 0).
+"Collapsed" / weak conditions and non-constant functions.  A constant pressure (ID 22) is the
+follower pressure of ksh_problem_set_pressure.
 
 Output mirrors constructDisplacement + patch evaluation at reference points
 (solvers/static_shell_XML.cpp:320-330) and gsStructuralAnalysisOutput's CSV layout
@@ -367,8 +368,8 @@ def read_problem(path: str, geometry: str | None = None, elevate: int = 0, refin
     # loads
     if 21 in ids:
         pt.fsurf = tuple(function_value(ids[21], 3))
-    if 22 in ids and np.any(function_value(ids[22], 1) != 0):
-        raise XmlError("follower pressure (ID 22) is not implemented")
+    if 22 in ids:  # follower pressure (static_shell_XML.cpp:145-149, 236-237: setPressure)
+        pt.pressure = float(function_value(ids[22], 1)[0])
     pts, loads = _matrix(ids.get(30), 2), _matrix(ids.get(31), 3)
     if pts.shape[1] != loads.shape[1]:
         raise XmlError("point load locations and vectors differ in number")
@@ -492,6 +493,8 @@ def write_problem(pt: g.Patch, fname: str, ref_points=None, options: dict | None
     f21 = ET.SubElement(root, "Function", type="FunctionExpr", id="21", tag="Loads", dim="3")
     for v in pt.fsurf:
         ET.SubElement(f21, "c").text = repr(float(v))
+    ET.SubElement(root, "Function", type="FunctionExpr", id="22", tag="Loads", dim="3").text = repr(
+        float(getattr(pt, "pressure", 0.0)))
     k = pt.point_xi.shape[0]
     ET.SubElement(root, "Matrix", rows="2", cols=str(k), id="30", tag="Loads").text = (
         " ".join(repr(float(v)) for v in pt.point_xi[:, 0]) + "\n" + " ".join(repr(float(v)) for v in pt.point_xi[:, 1]))

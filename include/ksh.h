@@ -85,6 +85,16 @@ int ksh_device_count(int *count);
 int ksh_problem_create(const ksh_patch *patch, const ksh_material *mat, const ksh_bc *bc,
                        const ksh_loads *loads, ksh_problem **out);
 int ksh_problem_destroy(ksh_problem *pb);
+/* Follower pressure p along a1 x a2 of the deformed surface (p (a1 x a2) dxi deta = p n da):
+ * gsThinShellAssembler::setPressure with a constant function
+ * (examples/gsThinShell_ArcLength.cpp:849,956-957; solvers/static_shell_XML.cpp:145-149,236-237).
+ * F becomes the dead loads plus the pressure's load on the undeformed surface P(0) (the drivers'
+ * Force = assemble() rhs, examples/gsThinShell_ArcLength.cpp:963-965); the residuals become
+ * r = F_int(u) - lambda F - (P(u) - P(0)) (ALM, the driver's Force - lambda Force - rhs(u),
+ * :997-1005) and its negative (static); the tangent K(u) - dP/du, of which the band keeps the
+ * lower triangle (what a lower-triangle LDL^T of the assembled matrix reads).  p = 0 removes it.
+ * Every engine and entry point that assembles K or R of the problem uses it. */
+int ksh_problem_set_pressure(ksh_problem *pb, double pressure);
 int ksh_num_free_dofs(const ksh_problem *pb, int64_t *n);
 int ksh_bandwidth(const ksh_problem *pb, int64_t *bw);
 int ksh_synchronize(ksh_problem *pb);

@@ -447,6 +447,103 @@ void ok_external_force(const ok_problem *pb, double *fext)
     free(Uy);
 }
 
+/*
+ * Follower pressure of one element: pe[3A+i] = p sum_g w_g N_A (a1 x a2)_i and
+ * Kpe[(3A+i)*ne + 3B+k] = d pe[3A+i] / d u_{B,k}
+ *                       = p sum_g w_g N_A (N_B,1 (e_k x a2) + N_B,2 (a1 x e_k))_i,
+ * with a1, a2 the deformed covariant base vectors and w_g the Gauss weight on the
+ * parameter domain (p (a1 x a2) dxi deta = p n da on the deformed surface).
+ */
+static void ok_pressure_element(const ok_problem *pb, const ok_ctx *c, const double *Ux, const double *Uy,
+                                const double *u_full, int ex, int ey, double *pe, double *Kpe)
+{
+    const int p = c->p, p1 = c->p1, nb = c->nb, ne = c->ne, nq = c->nq;
+    double X[OK_MAXNB][3];
+    for (int b = 0; b < p1; ++b)
+        for (int a = 0; a < p1; ++a) {
+            int A = a + p1 * b, cp = (ex + a) + pb->mx * (ey + b);
+            for (int i = 0; i < 3; ++i) X[A][i] = pb->ctrl[3 * cp + i] + (u_full ? u_full[3 * cp + i] : 0.0);
+        }
+    if (pe) memset(pe, 0, sizeof(double) * ne);
+    if (Kpe) memset(Kpe, 0, sizeof(double) * ne * ne);
+    for (int qy = 0; qy < nq; ++qy)
+        for (int qx = 0; qx < nq; ++qx) {
+            double xi = (ex + 0.5 * (1.0 + c->gx[qx])) / pb->nx;
+            double eta = (ey + 0.5 * (1.0 + c->gx[qy])) / pb->ny;
+            double wq = c->gw[qx] * c->gw[qy] * 0.25 / ((double)pb->nx * (double)pb->ny);
+            double dx[3 * OK_MAXP1], dy[3 * OK_MAXP1];
+            ok_ders_basis(p, Ux, ex + p, xi, 1, dx);
+            ok_ders_basis(p, Uy, ey + p, eta, 1, dy);
+            double N[OK_MAXNB], N1[OK_MAXNB], N2[OK_MAXNB];
+            for (int b = 0; b < p1; ++b)
+                for (int a = 0; a < p1; ++a) {
+                    int A = a + p1 * b;
+                    N[A] = dx[a] * dy[b];
+                    N1[A] = dx[p1 + a] * dy[b];
+                    N2[A] = dx[a] * dy[p1 + b];
+                }
+            double a1[3] = {0}, a2[3] = {0}, n[3];
+            for (int A = 0; A < nb; ++A)
+                for (int i = 0; i < 3; ++i) {
+                    a1[i] += N1[A] * X[A][i];
+                    a2[i] += N2[A] * X[A][i];
+                }
+            cross3(a1, a2, n);
+            const double s = pb->pressure * wq;
+            if (pe)
+                for (int A = 0; A < nb; ++A)
+                    for (int i = 0; i < 3; ++i) pe[3 * A + i] += s * N[A] * n[i];
+            if (Kpe)
+                for (int k = 0; k < 3; ++k) {
+                    double ek[3] = {0.0, 0.0, 0.0}, c2[3], c1[3];
+                    ek[k] = 1.0;
+                    cross3(ek, a2, c2); /* d(a1 x a2) / d a1_k */
+                    cross3(a1, ek, c1); /* d(a1 x a2) / d a2_k */
+                    for (int A = 0; A < nb; ++A)
+                        for (int i = 0; i < 3; ++i)
+                            for (int B = 0; B < nb; ++B)
+                                Kpe[(3 * A + i) * ne + 3 * B + k] += s * N[A] * (N1[B] * c2[i] + N2[B] * c1[i]);
+                }
+        }
+}
+
+void ok_pressure_force(const ok_problem *pb, const double *u_full, double *fp)
+{
+    ok_ctx c;
+    double *Ux, *Uy;
+    ctx_init(pb, &c, &Ux, &Uy);
+    int nel = pb->nx * pb->ny, ne = c.ne;
+    memset(fp, 0, sizeof(double) * 3 * pb->mx * pb->my);
+    double pe[OK_MAXNE];
+    for (int e = 0; e < nel; ++e) {
+        ok_pressure_element(pb, &c, Ux, Uy, u_full, e % pb->nx, e / pb->nx, pe, 0);
+        for (int r = 0; r < ne; ++r) fp[local_to_global(pb, c.p1, e % pb->nx, e / pb->nx, r)] += pe[r];
+    }
+    free(Ux);
+    free(Uy);
+}
+
+void ok_pressure_tangent_dense(const ok_problem *pb, const double *u_full, double *Kp)
+{
+    ok_ctx c;
+    double *Ux, *Uy;
+    ctx_init(pb, &c, &Ux, &Uy);
+    int nel = pb->nx * pb->ny, ne = c.ne, ndof = 3 * pb->mx * pb->my;
+    memset(Kp, 0, sizeof(double) * (size_t)ndof * ndof);
+    double *Kpe = (double *)malloc(sizeof(double) * ne * ne);
+    for (int e = 0; e < nel; ++e) {
+        int ex = e % pb->nx, ey = e / pb->nx;
+        ok_pressure_element(pb, &c, Ux, Uy, u_full, ex, ey, 0, Kpe);
+        for (int r = 0; r < ne; ++r)
+            for (int q = 0; q < ne; ++q)
+                Kp[(size_t)local_to_global(pb, c.p1, ex, ey, r) * ndof + local_to_global(pb, c.p1, ex, ey, q)] +=
+                    Kpe[r * ne + q];
+    }
+    free(Kpe);
+    free(Ux);
+    free(Uy);
+}
+
 static void assemble_matrix(const ok_problem *pb, const double *u_full, int which, const int *free_idx,
                             int nfree, int bw, double *out, int band)
 {

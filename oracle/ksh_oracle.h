@@ -45,6 +45,7 @@ typedef struct {
     const double *pl_xi; /* 2*npl parametric coordinates                  */
     const double *pl_F;  /* 3*npl load vectors                            */
     double fsurf[3];     /* dead surface force per unit undeformed area   */
+    double pressure;     /* follower pressure along a1 x a2 (0: none)     */
 } ok_problem;
 
 /* spline / quadrature primitives */
@@ -63,6 +64,11 @@ void   ok_internal_force(const ok_problem *pb, const double *u_full, double *fin
 void   ok_external_force(const ok_problem *pb, double *fext);
 void   ok_tangent_dense(const ok_problem *pb, const double *u_full, double *K);
 void   ok_mass_dense(const ok_problem *pb, double *M);
+/* follower pressure (gsThinShellAssembler::setPressure, examples/gsThinShell_ArcLength.cpp:956-957,
+ * solvers/static_shell_XML.cpp:236-237): P(u)_{A,i} = p sum_g w_g N_A (a1 x a2)_i on the deformed
+ * surface, and its derivative dP/du (full dofs, row-major ndof x ndof, not symmetric) */
+void   ok_pressure_force(const ok_problem *pb, const double *u_full, double *fp);
+void   ok_pressure_tangent_dense(const ok_problem *pb, const double *u_full, double *Kp);
 /* lower band of the free-dof tangent / mass: ab[J*(bw+1) + (I-J)] */
 void   ok_tangent_band(const ok_problem *pb, const int *free_idx, int nfree, int bw,
                        const double *u_full, double *ab);

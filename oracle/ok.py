@@ -25,7 +25,7 @@ class _Problem(C.Structure):
                 ("ctrl", C.POINTER(C.c_double)), ("t", C.c_double), ("E", C.c_double),
                 ("nu", C.c_double), ("rho", C.c_double), ("fixed", C.POINTER(C.c_int)),
                 ("npl", C.c_int), ("pl_xi", C.POINTER(C.c_double)), ("pl_F", C.POINTER(C.c_double)),
-                ("fsurf", C.c_double * 3)]
+                ("fsurf", C.c_double * 3), ("pressure", C.c_double)]
 
 
 def build():
@@ -51,6 +51,8 @@ def lib():
         L.ok_external_force.argtypes = [P, _dp]
         L.ok_tangent_dense.argtypes = [P, _dp, _dp]
         L.ok_mass_dense.argtypes = [P, _dp]
+        L.ok_pressure_force.argtypes = [P, _dp, _dp]
+        L.ok_pressure_tangent_dense.argtypes = [P, _dp, _dp]
         L.ok_tangent_band.argtypes = [P, _ip, C.c_int, C.c_int, _dp, _dp]
         L.ok_mass_band.argtypes = [P, _ip, C.c_int, C.c_int, _dp]
         L.ok_band_ldlt.argtypes = [C.c_int, C.c_int, _dp]
@@ -92,6 +94,7 @@ class Oracle:
         pb.pl_xi = self._keep[2].ctypes.data_as(C.POINTER(C.c_double))
         pb.pl_F = self._keep[3].ctypes.data_as(C.POINTER(C.c_double))
         pb.fsurf = (C.c_double * 3)(*patch.fsurf)
+        pb.pressure = float(getattr(patch, "pressure", 0.0))
         self.pb = pb
         self.ndof = patch.ndof
         self.free_idx = np.zeros(self.ndof, dtype=np.int32)
@@ -123,6 +126,22 @@ class Oracle:
         lib().ok_mass_dense(C.byref(self.pb), M)
         return M
 
+    @property
+    def pressure(self):
+        return self.pb.pressure
+
+    def pressure_force(self, u_full):
+        """P(u): the follower pressure p (a1 x a2) on the deformed surface (full dofs)."""
+        f = np.zeros(self.ndof)
+        lib().ok_pressure_force(C.byref(self.pb), np.ascontiguousarray(u_full, dtype=np.float64), f)
+        return f
+
+    def pressure_tangent_dense(self, u_full):
+        """dP/du on full dofs (not symmetric)."""
+        K = np.zeros((self.ndof, self.ndof))
+        lib().ok_pressure_tangent_dense(C.byref(self.pb), np.ascontiguousarray(u_full, dtype=np.float64), K)
+        return K
+
     # free-dof quantities (the solver-facing surface) --------------------------
     def to_full(self, u_free):
         u = np.zeros(self.ndof)
@@ -130,21 +149,37 @@ class Oracle:
         return u
 
     def force(self):
-        """F = assemble() rhs at zero displacement: the external load vector on free dofs."""
-        return self.external_force()[self.free]
+        """F = assemble() rhs at zero displacement: the external load vector on free dofs (with a
+        follower pressure, its value on the undeformed surface)."""
+        f = self.external_force()
+        if self.pressure != 0.0:
+            f = f + self.pressure_force(np.zeros(self.ndof))
+        return f[self.free]
 
     def residual_alm(self, u_free, lam):
-        """ALResidual: r = F_int(u) - lam F  (benchmark_Roof.cpp:335-344)."""
-        return self.internal_force(self.to_full(u_free))[self.free] - lam * self.force()
+        """ALResidual: r = F_int(u) - lam F  (benchmark_Roof.cpp:335-344).  With a follower pressure,
+        r = Force - lam Force - rhs(u) with rhs(u) = F_ext(u) - F_int(u) and Force = F_ext(0)
+        (examples/gsThinShell_ArcLength.cpp:997-1005): r = F_int(u) - lam F - (P(u) - P(0)), the
+        follower part at the full pressure whatever lam."""
+        u = self.to_full(u_free)
+        r = self.internal_force(u)[self.free] - lam * self.force()
+        if self.pressure != 0.0:
+            r = r - (self.pressure_force(u) - self.pressure_force(np.zeros(self.ndof)))[self.free]
+        return r
 
     def residual_static(self, u_free, lam=1.0):
         """Residual: R = lam F - F_int(u)  (nonlinear_shell_static.cpp:129-136)."""
         return -self.residual_alm(u_free, lam)
 
     def tangent_band(self, u_free):
+        """Lower band of the free-dof Jacobian d r / d u = K_int(u) - dP/du (with a follower
+        pressure not symmetric: the lower triangle is what a lower-triangle LDL^T reads)."""
         ab = np.zeros(self.nfree * (self.bw + 1))
-        lib().ok_tangent_band(C.byref(self.pb), self.free_idx, self.nfree, self.bw,
-                              np.ascontiguousarray(self.to_full(u_free)), ab)
+        u = np.ascontiguousarray(self.to_full(u_free))
+        lib().ok_tangent_band(C.byref(self.pb), self.free_idx, self.nfree, self.bw, u, ab)
+        if self.pressure != 0.0:
+            Kp = self.pressure_tangent_dense(u)[np.ix_(self.free, self.free)]
+            ab = ab - dense_to_band(Kp, self.bw)
         return ab
 
     def mass_band(self):

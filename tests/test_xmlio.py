@@ -20,6 +20,7 @@ def _same_patch(a, b, tol=1e-13):
     assert (a.t, a.E, a.nu, a.rho) == (b.t, b.E, b.nu, b.rho)
     assert np.allclose(a.point_xi, b.point_xi, rtol=0, atol=1e-15) and np.allclose(a.point_F, b.point_F, rtol=1e-15)
     assert tuple(a.fsurf) == tuple(b.fsurf)
+    assert a.pressure == b.pressure
 
 
 @pytest.mark.parametrize("refine", [2, 5])
@@ -33,9 +34,15 @@ def test_roof_file_reproduces_the_roof_builder(refine):
     assert kw["method"] == 0 and kw["tol_f"] == 1e-8 and kw["scaling"] == 0.0 and x.options[93]["Length"] == 2e-3
 
 
+def _pressurised_roof():
+    pt = g.roof_problem(p=3, n=4, P=0.0)
+    pt.pressure = -2.5
+    return pt
+
+
 @pytest.mark.parametrize("build", [lambda: g.clamped_plate_problem(p=2, n=6, q=3.0),
                                    lambda: g.compressed_strip_problem(nx=8),
-                                   lambda: g.roof_problem(p=3, n=4)])
+                                   lambda: g.roof_problem(p=3, n=4), _pressurised_roof])
 def test_write_then_read_round_trip(build, tmp_path):
     pt = build()
     f = str(tmp_path / "problem.xml")
