@@ -222,8 +222,9 @@ int ensure_rc64(Matrix &K);
 int band_solve_batch(hipStream_t s, void *d_descs, const std::vector<BandDescHost> &h, int prec);
 constexpr size_t kBandDescBytes = 48;
 int band_solve(Matrix &K, const void *fac, int prec, double *x_dev);
-// solve of one large band over many workgroups (ksh_solve_tiled.hip); z: n doubles of workspace
-int band_solve_tiled(hipStream_t s, const void *fac, int prec, int n, int bw, double *x, double *z);
+// solve of one large band over many workgroups (ksh_solve_tiled.hip); z: n doubles of workspace;
+// bar: 3 device unsigneds for the one-launch form's grid barrier (nullptr: stepped launches)
+int band_solve_tiled(hipStream_t s, const void *fac, int prec, int n, int bw, double *x, double *z, unsigned *bar);
 // path of a single-band factor / solve: 1 = multi-workgroup (tiled), 0 = one workgroup
 bool band_use_tiled_factor(int n, int bw, int prec);
 bool band_use_tiled_solve(int n, int bw, int prec);

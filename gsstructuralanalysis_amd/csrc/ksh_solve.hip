@@ -488,7 +488,8 @@ int band_solve(Matrix &K, const void *fac, int prec, double *x_dev)
     if (band_use_tiled_solve(K.n, K.bw, prec)) {
         ProfScope ps("band_solve_tiled", pb.stream);
         ps.set_work(2.0 * K.n * (K.bw + 1.0) * (prec == KSH_FP64 ? 8.0 : 4.0) + 16.0 * K.n);
-        return band_solve_tiled(pb.stream, fac, prec, K.n, K.bw, x_dev, K.work + 4 * (size_t)K.n);
+        return band_solve_tiled(pb.stream, fac, prec, K.n, K.bw, x_dev, K.work + 4 * (size_t)K.n,
+                                reinterpret_cast<unsigned *>(K.flag + 1));
     }
     const size_t smem = solve_lds(K.n, K.bw, (size_t)K.n * K.ld * (prec == KSH_FP64 ? sizeof(double) : sizeof(float)), prec);
     ProfScope ps("band_solve", pb.stream);
