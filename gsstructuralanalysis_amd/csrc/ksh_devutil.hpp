@@ -38,6 +38,14 @@ __device__ __forceinline__ int opaque(int v)
     return v;
 }
 
+// the same for a wave-uniform value (kept in an SGPR): values derived from it are recomputed where
+// they are used instead of hoisted out of a loop, each into a scalar register of its own
+__device__ __forceinline__ int opaque_s(int v)
+{
+    asm volatile("" : "+s"(v));
+    return v;
+}
+
 // Band reads and writes go through a buffer resource covering exactly the band (bytes < 2 GiB,
 // checked by the launchers).  A masked-off access is sent to offset 0x80000000, beyond the
 // resource: the buffer unit returns 0 for such a read and drops such a write.  Masked accesses

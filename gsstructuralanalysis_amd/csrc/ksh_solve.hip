@@ -9,18 +9,21 @@
 // launch covers a batch):
 //   * the right-hand side / solution vector stays in LDS for the whole solve, so HBM traffic is
 //     the band alone, read once per sweep: 2 n (bw+1) entries per solve;
-//   * rows are processed in blocks of 64, and the band entries a block needs are loaded into
-//     registers one block ahead (they do not depend on the solution), so their latency overlaps
-//     the reduction and the recurrence of the current block;
-//   * forward block r0..r0+63: lanes = rows, wave w of W takes band columns r0-bw+w+Wk; each load
-//     instruction reads 64 consecutive entries of one band column; partial products are summed
-//     over the waves in LDS, then wave 0 runs the 64-step recurrence with the diagonal block
-//     (staged in LDS), broadcasting through v_readlane;
-//   * backward block c0..c0+63: each wave owns 64/W consecutive columns; lanes run along each
-//     column's contiguous segment below the block, one wave reduction per column; wave 0 divides
-//     by the pivots and runs the transposed recurrence.
-// Band widths up to 512 (the prefetched columns per block); wider bands and vectors too long for
-// LDS take the streaming kernel below (vector in HBM).
+//   * rows are processed in blocks of 64 and the band entries a block needs are loaded into
+//     registers one block ahead (they do not depend on the solution);
+//   * a block's product with the solved part splits into "near" entries, which need the block
+//     solved just before, and "far" ones, which need only older blocks.  The far sums of the next
+//     block are formed while wave 0 runs this block's 64-step recurrence, so the recurrence is off
+//     the memory stream; only the near products (64 x 64 entries) sit between two recurrences;
+//   * forward block r0..r0+63: lanes = rows; each load instruction reads 64 consecutive entries of
+//     one band column; partial sums meet in LDS and wave 0 runs the recurrence with the diagonal
+//     block (staged in LDS by rows, read into registers ahead of use), broadcasting through
+//     v_readlane;
+//   * backward block c0..c0+63: each wave owns 64/W of the block's columns; lanes run along each
+//     column's contiguous segment below the block, one wave reduction per block; wave 0 divides by
+//     the pivots and runs the transposed recurrence (diagonal block staged by columns).
+// Band widths up to 512; wider bands and vectors too long for LDS take the streaming kernel below
+// (vector in HBM).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,25 +38,33 @@ namespace ksh {
 
 namespace {
 
-constexpr int kRB = 64;       // rows per block
-constexpr int kSdLd = kRB + 1;
+constexpr int kRB = 64;          // rows per block
+constexpr int kSdLd = kRB + 4;   // diagonal-block row stride: 16-byte aligned rows for vector reads
+constexpr int kBandMax = 512;    // widest band of the LDS-resident kernel
 
-// Workgroup shape per factor precision: NT threads, PF band entries per thread prefetched for
-// the next block (FP32: 1024 x 32, FP64: 512 x 64; both fit 128 / 256 VGPRs without spills and
-// prefetch 512 band columns per block).
+// Workgroup shape: 1024 threads (16 waves) for an FP32 factor, 512 for FP64 (its prefetch
+// registers fit the 256 VGPRs of two waves per SIMD, not the 128 of four).  Forward: wave w takes
+// NK of the 64 near columns (the previous block's) and FK far columns (offsets 65..512 before the
+// block).  Backward: wave w owns NK of the block's columns, one near row chunk (the next block's
+// rows) and BK far row chunks each.  FC: far registers consumed and refilled per scheduling
+// group; RC: diagonal-block entries per recurrence register run.
 template <typename T>
 struct SolveCfg {
     static constexpr int NT = sizeof(T) == 8 ? 512 : 1024;
-    static constexpr int W = NT / 64;          // waves
-    static constexpr int PF = 32 * 1024 / NT;  // prefetched entries per thread
-    static constexpr int CW = kRB / W;         // backward columns per wave
-    static constexpr int PC = PF / CW;         // prefetched backward loads per column
-    static constexpr int DQ = kRB * kRB / NT;  // diagonal-block entries per thread
+    static constexpr int W = NT / 64;
+    static constexpr int NK = kRB / W;
+    static constexpr int FK = (kBandMax - kRB) / W;
+    static constexpr int BK = kBandMax / kRB - 1;
+    static constexpr int DQ = kRB * kRB / NT;
+    static constexpr int FC = 7;
+    static constexpr int RC = sizeof(T) == 8 ? 8 : 16;
 };
+static_assert(SolveCfg<float>::FK % SolveCfg<float>::FC == 0 && SolveCfg<double>::FK % SolveCfg<double>::FC == 0, "far chunks");
+static_assert(kRB + SolveCfg<float>::W * SolveCfg<float>::FK == kBandMax, "far cover");
 
 // Zeros around the vector in LDS, so that the row and column indices of the band products need no
-// clamping: forward columns reach back to r0 - bw >= -512, backward rows up to c0 + 64 + 63 +
-// 64 (PC - 1) <= c0 + 575.
+// clamping: forward columns reach back to r0 + 64 - 514 >= -512, backward rows up to
+// c0 + 64 (BK + 1) + 63 <= c0 + 575.
 constexpr int kXPadF = 512;
 constexpr int kXPadB = 640;
 
@@ -91,24 +102,51 @@ __device__ __forceinline__ double wave_sums(const double (&a)[CW], int lane)
     return v[0];
 }
 
+typedef float ksh_f32x4 __attribute__((ext_vector_type(4)));
+typedef double ksh_f64x2 __attribute__((ext_vector_type(2)));
+
+// RC consecutive diagonal-block entries from LDS (16-byte aligned) into registers, 16 bytes a read
+template <int RC, typename T>
+__device__ __forceinline__ void lds_read_run(T (&dst)[RC], const T *src)
+{
+    if constexpr (sizeof(T) == 8) {
+#pragma unroll
+        for (int q = 0; q < RC / 2; ++q) {
+            const ksh_f64x2 v = reinterpret_cast<const ksh_f64x2 *>(src)[q];
+            dst[2 * q] = v.x;
+            dst[2 * q + 1] = v.y;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < RC / 4; ++q) {
+            const ksh_f32x4 v = reinterpret_cast<const ksh_f32x4 *>(src)[q];
+            dst[4 * q] = v.x;
+            dst[4 * q + 1] = v.y;
+            dst[4 * q + 2] = v.z;
+            dst[4 * q + 3] = v.w;
+        }
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ void band_solve_lds(const T *__restrict__ fac, int n, int bw, double *__restrict__ xg, unsigned char *smem)
 {
-    const size_t ld = (size_t)bw + 1;
-    const unsigned ldu = (unsigned)bw + 1;  // n * ld * sizeof(T) < 4 GiB (checked by the launcher)
-    const __amdgpu_buffer_rsrc_t rs = band_rsrc(fac, (unsigned)((size_t)n * ld * sizeof(T)));
-    constexpr int NT = SolveCfg<T>::NT, kWaves = SolveCfg<T>::W, kPF = SolveCfg<T>::PF;
-    constexpr int CW = SolveCfg<T>::CW, DQ = SolveCfg<T>::DQ;
+    using C = SolveCfg<T>;
+    constexpr int NT = C::NT, W = C::W, NK = C::NK, FK = C::FK, BK = C::BK, DQ = C::DQ, FC = C::FC, RC = C::RC;
+    const unsigned ldu = (unsigned)bw + 1;  // n * ld * sizeof(T) < 2 GiB (checked by the launcher)
+    const __amdgpu_buffer_rsrc_t rs = band_rsrc(fac, (unsigned)((size_t)n * ldu * sizeof(T)));
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wv = __builtin_amdgcn_readfirstlane(wave);
+    const unsigned lane_off = (unsigned)lane * (unsigned)sizeof(T);
     double *xs = (double *)smem + kXPadF;
-    double *sp = xs + ((n + 1) & ~1) + kXPadB;  // [kWaves][64] partial sums
-    T *sd = (T *)(sp + kWaves * kRB);        // [64][65]: sd[jj*65 + ii] = L[r0+ii][r0+jj]
+    double *sp = xs + ((n + 1) & ~1) + kXPadB;  // [W][64] partial sums
+    T *sd = (T *)(sp + W * kRB);                // diagonal block, 64 runs of kSdLd
     const int nblk = (n + kRB - 1) / kRB;
-    T pf[kPF];
     T dg[DQ];
 
-    // diagonal block of rows/columns r0..r0+63 (strict lower part, or with the pivots)
+    // diagonal block of rows/columns r0..r0+63 (strict lower part, or with the pivots): thread entry
+    // (ii, jj) = L[r0+ii][r0+jj], stored by rows (sd[ii][jj], the forward recurrence's operands of
+    // lane ii) or by columns (sd[jj][ii], the backward one's)
     auto diag_load = [&](int r0, bool with_d) {
 #pragma unroll
         for (int q = 0; q < DQ; ++q) {
@@ -118,12 +156,18 @@ __device__ __forceinline__ void band_solve_lds(const T *__restrict__ fac, int n,
             dg[q] = load_if<T>(ok, rs, (unsigned)(((r0 + jj) * ldu + (ii - jj)) * sizeof(T)));
         }
     };
-    auto diag_store = [&]() {
+    auto diag_store = [&](bool by_rows) {
 #pragma unroll
         for (int q = 0; q < DQ; ++q) {
             const int idx = tid + NT * q;
-            sd[(idx >> 6) * kSdLd + (idx & 63)] = dg[q];
+            const int jj = idx >> 6, ii = idx & 63;
+            sd[by_rows ? ii * kSdLd + jj : jj * kSdLd + ii] = dg[q];
         }
+    };
+    // 64 band entries at element base + lane; lanes above lim (and every lane when lim < 0) read
+    // zero through an out-of-range offset.  base and lim are wave-uniform.
+    auto band_load = [&](int lim, int base) -> T {
+        return load_vs<T>(rs, lane <= lim ? lane_off : kMaskedOff, lim >= 0 ? (unsigned)base * (unsigned)sizeof(T) : 0u);
     };
 
     for (int i = tid; i < n; i += NT) xs[i] = xg[i];
@@ -131,117 +175,158 @@ __device__ __forceinline__ void band_solve_lds(const T *__restrict__ fac, int n,
     for (int i = n + tid; i < ((n + 1) & ~1) + kXPadB; i += NT) xs[i] = 0.0;
 
     // ---- forward: L y = b ----
-    // Entry (i, j), i = r0 + lane, j = r0 - bw + wv + W k (one band column per wave and k) is at
-    // element j (bw + 1) + i - j = j bw + r0 + lane: a per-lane part plus a wave-uniform part,
-    // which the scalar unit computes (kMaskedOff for a column outside [0, r0)).  Per-lane masks
-    // are needed only where rows leave the band (i - j > bw: the first columns, lane > wv + W k)
-    // or the matrix (the last block).
-    constexpr int kCh = 8;  // prefetch registers are consumed and refilled in chunks of 8
-    auto fwd_load = [&](int r0, int c) {
-        const bool row_ok = r0 + lane < n;
+    // Entry (r0 + lane, j), j = r0 - off, is element j (bw + 1) + r0 + lane - j = j bw + r0 + lane;
+    // lanes with lane + off > bw (outside the band) or r0 + lane >= n read zero.
+    {
+        // (bb: the band width; the FP64 kernel passes it per scheduling group through opaque_s, so
+        // that its per-load offsets are formed where they are used rather than all held in SGPRs
+        // across the loop, which its register budget cannot afford)
+        auto fwd_entry = [&](int r0, int off, int bb) -> T {
+            const int j = r0 - off;
+            return band_load(j >= 0 ? min(bb - off, n - 1 - r0) : -1, j * bb + r0);
+        };
+        // near offsets 1 + w + W k (k < NK) cover 1..64, far offsets 65 + w + W k (k < FK)
+        // 65..512.  Blocks 0 and 1 have no entries there (the loads read zeros); they are issued
+        // anyway, in the loop's order (near, diagonal block, far), so that the waits the compiler
+        // derives for the loop, which also hold on its entry, are the steady-state ones.
+        T fn[NK], ff[FK];
 #pragma unroll
-        for (int u = 0; u < kCh; ++u) {
-            const int k = kCh * c + u;
-            const int j = r0 - bw + wv + kWaves * k;
-            const unsigned so = (j >= 0 && j < r0) ? (unsigned)((j * bw + r0) * (int)sizeof(T)) : kMaskedOff;
-            const unsigned off = (unsigned)lane * (unsigned)sizeof(T) + so;
-            // lane > wv + W k (i - j > bw) only happens for W k < 64 (a compile-time condition)
-            const bool ok = kWaves * k < 64 ? row_ok && lane <= wv + kWaves * k : row_ok;
-            pf[k] = load_vs<T>(rs, ok ? off : kMaskedOff, 0);
-        }
-    };
+        for (int k = 0; k < NK; ++k) fn[k] = fwd_entry(0, 1 + wv + W * k, bw);
+        __builtin_amdgcn_sched_barrier(0);
+        diag_load(0, false);
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int c = 0; c < kPF / kCh; ++c) fwd_load(0, c);
-    diag_load(0, false);
-    lds_barrier();
-    for (int b = 0; b < nblk; ++b) {
-        const int r0 = b * kRB, i = r0 + lane;
-        double acc = 0.0;
-        // consume a chunk of this block's entries, then refill those registers with the next
-        // block's (the scheduling barrier keeps the compiler from hoisting every LDS read and
-        // load at once, which would exceed the register budget)
-#pragma unroll
-        for (int c = 0; c < kPF / kCh; ++c) {
-            const double *xw = xs + (r0 - bw + wv);  // column j of k at xw[W k] (zeros before x)
-#pragma unroll
-            for (int u = 0; u < kCh; ++u) {
-                const int k = kCh * c + u;
-                acc += (double)pf[k] * xw[kWaves * k];
-            }
-            fwd_load(r0 + kRB, c);  // past the last block every read is masked off
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        sp[wave * kRB + lane] = acc;
-        diag_store();
-        diag_load(r0 + kRB, false);
+        for (int k = 0; k < FK; ++k) ff[k] = fwd_entry(kRB, kRB + 1 + wv + W * k, bw);
+        double facc = 0.0;  // far sum of the current block
         lds_barrier();
-        if (wave == 0) {
-            double s = 0.0;
+        for (int b = 0; b < nblk; ++b) {
+            const int r0 = b * kRB, i = r0 + lane;
+            double acc = facc;
 #pragma unroll
-            for (int w = 0; w < kWaves; ++w) s += sp[w * kRB + lane];
-            double y = i < n ? xs[i] - s : 0.0;
-#pragma unroll 8
-            for (int jj = 0; jj < kRB - 1; ++jj) {
-                // sd holds zeros on and above the diagonal, so no lane mask is needed
-                const double l = (double)sd[jj * kSdLd + lane];
-                y -= l * readlane_f64(y, jj);
+            for (int k = 0; k < NK; ++k) acc += (double)fn[k] * xs[r0 - 1 - wv - W * k];
+            sp[wave * kRB + lane] = acc;
+#pragma unroll
+            for (int k = 0; k < NK; ++k) fn[k] = fwd_entry(r0 + kRB, 1 + wv + W * k, bw);
+            diag_store(true);
+            diag_load(r0 + kRB, false);
+            lds_barrier();
+            if (wv == 0) {
+                double s = 0.0;
+#pragma unroll
+                for (int w = 0; w < W; ++w) s += sp[w * kRB + lane];
+                double y = i < n ? xs[i] - s : 0.0;
+                // y_i -= L[i][jj] y_jj for jj = 0..62; L[i][jj] = 0 for jj >= i in the staged block
+                const T *row = sd + lane * kSdLd;
+                T lr[2][RC];
+                lds_read_run<RC>(lr[0], row);
+#pragma unroll
+                for (int q = 0; q < kRB / RC; ++q) {
+                    if (q + 1 < kRB / RC) lds_read_run<RC>(lr[(q + 1) & 1], row + RC * (q + 1));
+#pragma unroll
+                    for (int t = 0; t < RC; ++t) {
+                        const int jj = RC * q + t;
+                        if (jj < kRB - 1) y -= (double)lr[q & 1][t] * readlane_f64(y, jj);
+                    }
+                }
+                if (i < n) xs[i] = y;
             }
-            if (i < n) xs[i] = y;
+            // far sum of block b + 1: columns r0 - 1 - w - W k, all solved (wave 0 after its
+            // recurrence)
+            const double *xw = xs + (r0 - 1 - wv);
+            double a = 0.0;
+#pragma unroll
+            for (int c = 0; c < FK / FC; ++c) {
+#pragma unroll
+                for (int u = 0; u < FC; ++u) {
+                    const int k = FC * c + u;
+                    a += (double)ff[k] * xw[-W * k];
+                }
+                const int bb = sizeof(T) == 8 ? opaque_s(bw) : bw;
+#pragma unroll
+                for (int u = 0; u < FC; ++u) {
+                    const int k = FC * c + u;
+                    ff[k] = fwd_entry(r0 + 2 * kRB, kRB + 1 + wv + W * k, bb);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            facc = a;
+            lds_barrier();
         }
-        lds_barrier();
     }
 
     // ---- diagonal and backward: L^T x = D^-1 y ----
-    constexpr int kPC = SolveCfg<T>::PC;  // prefetched loads per column
-    // column j = c0 + jj, rows j + d, d = (64 - jj) + 64 k + lane: element j (bw + 1) + d, a
-    // wave-uniform part plus lane; lanes at or past lim (d > bw or j + d >= n) are masked
-    auto bwd_load = [&](int c0, int c) {
-        const int jj = CW * wv + c, j = c0 + jj;
-        const bool cok = c0 >= 0 && j < n;
+    // Wave w owns columns j = c0 + jj, jj = NK w + c.  Row chunk k of a column holds rows
+    // c0 + 64 (k + 1) + lane, element j (bw + 1) + dl + lane with dl = 64 - jj + 64 k; lanes past
+    // the band (dl + lane > bw) or the matrix read zero.  Chunk 0 is near (the next block's rows),
+    // chunks 1..BK far; the far sums of block b - 1 are formed while wave 0 solves block b.
+    {
+        auto bwd_entry = [&](int c0, int jj, int k, int bb) -> T {
+            const int j = c0 + jj, dl = kRB - jj + kRB * k;
+            return band_load(c0 >= 0 && j < n ? min(bb - dl, n - 1 - j - dl) : -1, j * (bb + 1) + dl);
+        };
+        // the last block has no rows below it, the one before none beyond its near chunk: their
+        // loads read zeros, issued in the loop's order as in the forward sweep
+        const int cl = (nblk - 1) * kRB;
+        T bn[NK], bf[NK][BK];
+        double bacc[NK];
 #pragma unroll
-        for (int k = 0; k < kPC; ++k) {
-            const int dl = (kRB - jj) + 64 * k;
-            const unsigned so = cok ? (unsigned)((j * (int)ldu + dl) * (int)sizeof(T)) : kMaskedOff;
-            unsigned off = (unsigned)lane * (unsigned)sizeof(T) + so;
-            const int lim = min(bw - dl + 1, n - j - dl);
-            pf[c * kPC + k] = load_vs<T>(rs, lane < lim ? off : kMaskedOff, 0);
+        for (int c = 0; c < NK; ++c) {
+            bn[c] = bwd_entry(cl, NK * wv + c, 0, bw);
+            bacc[c] = 0.0;
         }
-    };
+        __builtin_amdgcn_sched_barrier(0);
+        diag_load(cl, true);
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int c = 0; c < CW; ++c) bwd_load((nblk - 1) * kRB, c);
-    diag_load((nblk - 1) * kRB, true);
-    for (int b = nblk - 1; b >= 0; --b) {
-        const int c0 = b * kRB;
-        double a[CW];
-        const double *xb = xs + c0 + kRB + lane;  // row j + d of k at xb[64 k] (zeros after x)
+        for (int c = 0; c < NK; ++c)
 #pragma unroll
-        for (int c = 0; c < CW; ++c) {
-            double acc = 0.0;
+            for (int k = 0; k < BK; ++k) bf[c][k] = bwd_entry(cl - kRB, NK * wv + c, k + 1, bw);
+        for (int b = nblk - 1; b >= 0; --b) {
+            const int c0 = b * kRB, i = c0 + lane;
+            double a[NK];
 #pragma unroll
-            for (int k = 0; k < kPC; ++k) acc += (double)pf[c * kPC + k] * xb[64 * k];
-            a[c] = acc;
-            bwd_load(c0 - kRB, c);  // before the first block every read is masked off
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        const double s = wave_sums<CW>(a, lane);
-        if ((lane & (64 / CW - 1)) == 0) sp[CW * wave + lane / (64 / CW)] = s;
-        diag_store();
-        diag_load(c0 - kRB, true);
-        lds_barrier();
-        if (wave == 0) {
-            const int i = c0 + lane;
-            const double dv = (double)sd[lane * kSdLd + lane];
-            sd[lane * kSdLd + lane] = T(0);  // only this lane reads its diagonal entry
-            double xv = i < n ? xs[i] / dv - sp[lane] : 0.0;
-#pragma unroll 8
-            for (int jj = kRB - 1; jj > 0; --jj) {
-                // zeros below the diagonal (and the cleared pivot) make the lane mask unnecessary
-                const double l = (double)sd[lane * kSdLd + jj];
-                xv -= l * readlane_f64(xv, jj);
+            for (int c = 0; c < NK; ++c) a[c] = bacc[c] + (double)bn[c] * xs[c0 + kRB + lane];
+            const double s = wave_sums<NK>(a, lane);
+            if ((lane & (64 / NK - 1)) == 0) sp[NK * wave + lane / (64 / NK)] = s;
+#pragma unroll
+            for (int c = 0; c < NK; ++c) bn[c] = bwd_entry(c0 - kRB, NK * wv + c, 0, bw);
+            diag_store(false);
+            diag_load(c0 - kRB, true);
+            lds_barrier();
+            if (wv == 0) {
+                T *col = sd + lane * kSdLd;  // col[jj] = L[c0+jj][c0+lane]
+                const double dv = (double)col[lane];
+                col[lane] = T(0);  // only this lane reads its pivot
+                double xv = i < n ? xs[i] / dv - sp[lane] : 0.0;
+                // x_i -= L[jj][i] x_jj for jj = 63..1; zeros above the diagonal and the cleared pivot
+                T lr[2][RC];
+                lds_read_run<RC>(lr[0], col + kRB - RC);
+#pragma unroll
+                for (int q = 0; q < kRB / RC; ++q) {
+                    const int q0 = kRB - RC * (q + 1);  // first entry of this run
+                    if (q + 1 < kRB / RC) lds_read_run<RC>(lr[(q + 1) & 1], col + q0 - RC);
+#pragma unroll
+                    for (int t = RC - 1; t >= 0; --t) {
+                        const int jj = q0 + t;
+                        if (jj > 0) xv -= (double)lr[q & 1][t] * readlane_f64(xv, jj);
+                    }
+                }
+                if (i < n) xs[i] = xv;
             }
-            if (i < n) xs[i] = xv;
+            // far sums of block b - 1: rows c0 + 64 k + lane, k >= 1, all solved
+#pragma unroll
+            for (int c = 0; c < NK; ++c) {
+                double f = 0.0;
+#pragma unroll
+                for (int k = 0; k < BK; ++k) f += (double)bf[c][k] * xs[c0 + kRB * (k + 1) + lane];
+                bacc[c] = f;
+                const int bb = sizeof(T) == 8 ? opaque_s(bw) : bw;
+#pragma unroll
+                for (int k = 0; k < BK; ++k) bf[c][k] = bwd_entry(c0 - 2 * kRB, NK * wv + c, k + 1, bb);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            lds_barrier();
         }
-        lds_barrier();
     }
     for (int i = tid; i < n; i += NT) xg[i] = xs[i];
 }
@@ -404,13 +489,12 @@ int allow_solve_lds(size_t need)
 
 // LDS bytes of the LDS-resident kernel for a batch with largest order nmax and band width bwmax,
 // or 0 when it does not apply: the vector must fit in LDS, the band width must not exceed the
-// prefetched columns per block (512 for both precisions), and band byte offsets must stay below
+// columns the block prefetches cover (512 for both precisions), and band byte offsets must stay below
 // the 2 GiB masked-read offset.
 size_t solve_lds(int nmax, int bwmax, size_t bytes_max, int prec)
 {
-    const int cols = prec == KSH_FP64 ? SolveCfg<double>::W * SolveCfg<double>::PF : SolveCfg<float>::W * SolveCfg<float>::PF;
     const size_t smem = prec == KSH_FP64 ? solve_lds_bytes<double>(nmax) : solve_lds_bytes<float>(nmax);
-    if (smem > kSolveLdsMax || bwmax > cols || bwmax > 64 * SolveCfg<float>::PC || bytes_max >= 0x7F000000ull) return 0;
+    if (smem > kSolveLdsMax || bwmax > kBandMax || bytes_max >= 0x7F000000ull) return 0;
     return smem;
 }
 
