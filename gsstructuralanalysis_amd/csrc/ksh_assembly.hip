@@ -21,6 +21,8 @@
 // thread to (instance, dof / band slot) by binary search over the prefix offsets.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <algorithm>
 #include <cmath>
 
@@ -116,32 +118,70 @@ __global__ __launch_bounds__(128) void k_residual_gp(const InstDesc *descs, int 
 }
 
 // Residual, stage 2: one wave per element; lane (A, i) sums Fe[3A+i] = sum_g sum_a
-// Psi_g[a][A] f_g[3a+i] (g outer, a inner).
+// Psi_g[a][A] f_g[3a+i] (g outer, a inner).  The wave first stages its element's Gauss-point
+// resultants (kFq or, with a follower pressure, kFqP rows of NG contiguous values) and its two
+// basis tables (NQ x 3 x P1 values each, contiguous per element column / row) in LDS, every load
+// in flight at once; the sums then read LDS only.  (Reading them inside the sum cost one memory
+// latency per Gauss point.)
 template <int P>
 __global__ __launch_bounds__(256) void k_residual_tiles(const InstDesc *descs, int ninst, int total_el)
 {
-    constexpr int P1 = P + 1, NB = P1 * P1, NQ = P1, NG = NQ * NQ;
-    const int eg = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    constexpr int P1 = P + 1, NB = P1 * P1, NQ = P1, NG = NQ * NQ, NT = NQ * 3 * P1;
+    constexpr int FL = (kFqP * NG + 63) / 64, TL = (2 * NT + 63) / 64;  // loads per lane
+    __shared__ double sF[4][kFqP * NG];
+    __shared__ double sT[4][2 * NT];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int eg = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + w));
     if (eg >= total_el) return;
     const InstDesc &d = descs[find_inst(ninst, eg, [&](int m) { return (long long)descs[m].elem0; })];
     const int e = eg - d.elem0, ex = e % d.nx, ey = e / d.nx;
-    const int lane = threadIdx.x & 63;
-    if (lane >= 3 * NB) return;
-    const int A = lane / 3, i = lane % 3, ia = A % P1, ib = A / P1;
+    const bool prs = d.pressure != 0.0;
+    const int nf = (prs ? kFqP : kFq) * NG;
     const size_t stride = (size_t)d.nel * NG;
     const double *Fg = d.Hg + (size_t)e * NG;
-    double acc = 0.0;
-    for (int g = 0; g < NG; ++g) {
-        const int qx = g % NQ, qy = g / NQ;
-        const double *tx = d.tabx + ((size_t)(ex * NQ + qx) * 3) * P1;
-        const double *ty = d.taby + ((size_t)(ey * NQ + qy) * 3) * P1;
-        const double psi[5] = {tx[P1 + ia] * ty[ib], tx[ia] * ty[P1 + ib], tx[2 * P1 + ia] * ty[ib],
-                               tx[ia] * ty[2 * P1 + ib], tx[P1 + ia] * ty[P1 + ib]};
+    double vf[FL], vt[TL];
 #pragma unroll
-        for (int a = 0; a < 5; ++a) acc += psi[a] * Fg[(size_t)(3 * a + i) * stride + g];
-        if (d.pressure != 0.0) acc += tx[ia] * ty[ib] * Fg[(size_t)(kFq + i) * stride + g];
+    for (int q = 0; q < FL; ++q) {
+        const int idx = lane + 64 * q;
+        vf[q] = idx < nf ? Fg[(size_t)(idx / NG) * stride + idx % NG] : 0.0;
     }
-    d.Fe[(size_t)e * 3 * NB + lane] = acc;
+#pragma unroll
+    for (int q = 0; q < TL; ++q) {
+        const int idx = lane + 64 * q;
+        vt[q] = idx < NT ? d.tabx[(size_t)ex * NT + idx] : idx < 2 * NT ? d.taby[(size_t)ey * NT + idx - NT] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < FL; ++q)
+        if (lane + 64 * q < kFqP * NG) sF[w][lane + 64 * q] = vf[q];
+#pragma unroll
+    for (int q = 0; q < TL; ++q)
+        if (lane + 64 * q < 2 * NT) sT[w][lane + 64 * q] = vt[q];
+    // the wave reads what its other lanes wrote: LDS accesses of one wave complete in order, and
+    // the fence keeps the compiler from moving the reads above the writes
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+    if (lane >= 3 * NB) return;
+    const int A = lane / 3, i = lane % 3, ia = A % P1, ib = A / P1;
+    const double *F = sF[w];
+    // the follower-pressure term is a separate instantiation of the loop, so that the loop without
+    // it keeps no branch
+    auto sum = [&](auto with_pressure) {
+        double acc = 0.0;
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            const int qx = g % NQ, qy = g / NQ;
+            const double *tx = sT[w] + qx * 3 * P1;
+            const double *ty = sT[w] + NT + qy * 3 * P1;
+            const double psi[5] = {tx[P1 + ia] * ty[ib], tx[ia] * ty[P1 + ib], tx[2 * P1 + ia] * ty[ib],
+                                   tx[ia] * ty[2 * P1 + ib], tx[P1 + ia] * ty[P1 + ib]};
+#pragma unroll
+            for (int a = 0; a < 5; ++a) acc += psi[a] * F[(3 * a + i) * NG + g];
+            if constexpr (decltype(with_pressure)::value) acc += tx[ia] * ty[ib] * F[(kFq + i) * NG + g];
+        }
+        return acc;
+    };
+    d.Fe[(size_t)e * 3 * NB + lane] = prs ? sum(std::true_type{}) : sum(std::false_type{});
 }
 
 // Tangent, stage 1: one thread per Gauss point.  Derivative vectors of the deformed surface,
