@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cstdlib>
 #include <cstring>
 
@@ -128,12 +129,34 @@ __device__ __forceinline__ void lds_read_run(T (&dst)[RC], const T *src)
     }
 }
 
+// G band loads of 64 consecutive entries: load u reads elements base + u db + lane, and lanes above
+// lim_u = min(e + u de, rl, jl + u djl) read zero (an out-of-range offset).  The scalar state is
+// advanced by one add per term and load, so a load costs about six scalar instructions; a load
+// whose column lies before the matrix (base < 0) has lim < 0 and byte offset 0.
+template <int G, typename T>
+__device__ __forceinline__ void band_run(T *dst, __amdgpu_buffer_rsrc_t rs, int lane, unsigned lane_off, int base, int db, int e, int de,
+                                         int rl, int jl, int djl)
+{
+    int sb = base * (int)sizeof(T);
+    const int dsb = db * (int)sizeof(T);
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+        // (two scalar mins: opaque_s keeps the compiler from fusing them into a vector min3)
+        const int lim = min(opaque_s(min(e, rl)), jl);
+        dst[u] = load_vs<T>(rs, lane <= lim ? lane_off : kMaskedOff, (unsigned)max(sb, 0));
+        sb += dsb;
+        e += de;
+        jl += djl;
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ void band_solve_lds(const T *__restrict__ fac, int n, int bw, double *__restrict__ xg, unsigned char *smem)
 {
     using C = SolveCfg<T>;
     constexpr int NT = C::NT, W = C::W, NK = C::NK, FK = C::FK, BK = C::BK, DQ = C::DQ, FC = C::FC, RC = C::RC;
     const unsigned ldu = (unsigned)bw + 1;  // n * ld * sizeof(T) < 2 GiB (checked by the launcher)
+    const int ld = bw + 1;
     const __amdgpu_buffer_rsrc_t rs = band_rsrc(fac, (unsigned)((size_t)n * ldu * sizeof(T)));
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wv = __builtin_amdgcn_readfirstlane(wave);
@@ -164,11 +187,6 @@ __device__ __forceinline__ void band_solve_lds(const T *__restrict__ fac, int n,
             sd[by_rows ? ii * kSdLd + jj : jj * kSdLd + ii] = dg[q];
         }
     };
-    // 64 band entries at element base + lane; lanes above lim (and every lane when lim < 0) read
-    // zero through an out-of-range offset.  base and lim are wave-uniform.
-    auto band_load = [&](int lim, int base) -> T {
-        return load_vs<T>(rs, lane <= lim ? lane_off : kMaskedOff, lim >= 0 ? (unsigned)base * (unsigned)sizeof(T) : 0u);
-    };
 
     for (int i = tid; i < n; i += NT) xs[i] = xg[i];
     for (int i = tid; i < kXPadF; i += NT) xs[i - kXPadF] = 0.0;
@@ -178,25 +196,23 @@ __device__ __forceinline__ void band_solve_lds(const T *__restrict__ fac, int n,
     // Entry (r0 + lane, j), j = r0 - off, is element j (bw + 1) + r0 + lane - j = j bw + r0 + lane;
     // lanes with lane + off > bw (outside the band) or r0 + lane >= n read zero.
     {
-        // (bb: the band width; the FP64 kernel passes it per scheduling group through opaque_s, so
-        // that its per-load offsets are formed where they are used rather than all held in SGPRs
-        // across the loop, which its register budget cannot afford)
-        auto fwd_entry = [&](int r0, int off, int bb) -> T {
-            const int j = r0 - off;
-            return band_load(j >= 0 ? min(bb - off, n - 1 - r0) : -1, j * bb + r0);
+        // G loads at offsets off0, off0 + W, ...: column j = r0 - off, lanes up to bw - off (the
+        // band) and n - 1 - r0 (the matrix); j < 0 gives jl = 64 j + 63 < 0
+        auto fwd_run = [&](T *dst, auto g, int r0, int off0) {
+            constexpr int G = decltype(g)::value;
+            const int j0 = r0 - off0;
+            band_run<G, T>(dst, rs, lane, lane_off, j0 * bw + r0, -W * bw, bw - off0, -W, n - 1 - r0, 64 * j0 + 63, -64 * W);
         };
         // near offsets 1 + w + W k (k < NK) cover 1..64, far offsets 65 + w + W k (k < FK)
         // 65..512.  Blocks 0 and 1 have no entries there (the loads read zeros); they are issued
         // anyway, in the loop's order (near, diagonal block, far), so that the waits the compiler
         // derives for the loop, which also hold on its entry, are the steady-state ones.
         T fn[NK], ff[FK];
-#pragma unroll
-        for (int k = 0; k < NK; ++k) fn[k] = fwd_entry(0, 1 + wv + W * k, bw);
+        fwd_run(fn, std::integral_constant<int, NK>{}, 0, 1 + wv);
         __builtin_amdgcn_sched_barrier(0);
         diag_load(0, false);
         __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int k = 0; k < FK; ++k) ff[k] = fwd_entry(kRB, kRB + 1 + wv + W * k, bw);
+        fwd_run(ff, std::integral_constant<int, FK>{}, kRB, kRB + 1 + wv);
         double facc = 0.0;  // far sum of the current block
         lds_barrier();
         for (int b = 0; b < nblk; ++b) {
@@ -205,8 +221,7 @@ __device__ __forceinline__ void band_solve_lds(const T *__restrict__ fac, int n,
 #pragma unroll
             for (int k = 0; k < NK; ++k) acc += (double)fn[k] * xs[r0 - 1 - wv - W * k];
             sp[wave * kRB + lane] = acc;
-#pragma unroll
-            for (int k = 0; k < NK; ++k) fn[k] = fwd_entry(r0 + kRB, 1 + wv + W * k, bw);
+            fwd_run(fn, std::integral_constant<int, NK>{}, r0 + kRB, 1 + wv);
             diag_store(true);
             diag_load(r0 + kRB, false);
             lds_barrier();
@@ -241,12 +256,7 @@ __device__ __forceinline__ void band_solve_lds(const T *__restrict__ fac, int n,
                     const int k = FC * c + u;
                     a += (double)ff[k] * xw[-W * k];
                 }
-                const int bb = sizeof(T) == 8 ? opaque_s(bw) : bw;
-#pragma unroll
-                for (int u = 0; u < FC; ++u) {
-                    const int k = FC * c + u;
-                    ff[k] = fwd_entry(r0 + 2 * kRB, kRB + 1 + wv + W * k, bb);
-                }
+                fwd_run(ff + FC * c, std::integral_constant<int, FC>{}, r0 + 2 * kRB, kRB + 1 + wv + W * FC * c);
                 __builtin_amdgcn_sched_barrier(0);
             }
             facc = a;
@@ -260,27 +270,32 @@ __device__ __forceinline__ void band_solve_lds(const T *__restrict__ fac, int n,
     // the band (dl + lane > bw) or the matrix read zero.  Chunk 0 is near (the next block's rows),
     // chunks 1..BK far; the far sums of block b - 1 are formed while wave 0 solves block b.
     {
-        auto bwd_entry = [&](int c0, int jj, int k, int bb) -> T {
-            const int j = c0 + jj, dl = kRB - jj + kRB * k;
-            return band_load(c0 >= 0 && j < n ? min(bb - dl, n - 1 - j - dl) : -1, j * (bb + 1) + dl);
+        // near chunk of the wave's NK columns jj = NK w + u: element (c0 + jj) ld + 64 - jj + lane,
+        // lanes up to bw - 64 + jj (the band) and n - 1 - c0 - 64 (the matrix)
+        auto bwd_near = [&](T (&dst)[NK], int c0) {
+            const int jj0 = NK * wv;
+            band_run<NK, T>(dst, rs, lane, lane_off, (c0 + jj0) * ld + kRB - jj0, bw, bw - kRB + jj0, 1, n - 1 - c0 - kRB,
+                            c0 >= 0 ? 63 : -1, 0);
+        };
+        // far chunks 1..BK of column j = c0 + jj: element j ld + dl + lane, dl = 64 (k + 1) - jj, lanes
+        // up to min(bw, n - 1 - j) - dl
+        auto bwd_far = [&](T (&dst)[BK], int c0, int jj) {
+            const int j = c0 + jj, dl = 2 * kRB - jj;
+            band_run<BK, T>(dst, rs, lane, lane_off, j * ld + dl, kRB, min(bw, n - 1 - j) - dl, -kRB, 1 << 30, c0 >= 0 ? 63 : -1, 0);
         };
         // the last block has no rows below it, the one before none beyond its near chunk: their
         // loads read zeros, issued in the loop's order as in the forward sweep
         const int cl = (nblk - 1) * kRB;
         T bn[NK], bf[NK][BK];
         double bacc[NK];
+        bwd_near(bn, cl);
 #pragma unroll
-        for (int c = 0; c < NK; ++c) {
-            bn[c] = bwd_entry(cl, NK * wv + c, 0, bw);
-            bacc[c] = 0.0;
-        }
+        for (int c = 0; c < NK; ++c) bacc[c] = 0.0;
         __builtin_amdgcn_sched_barrier(0);
         diag_load(cl, true);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int c = 0; c < NK; ++c)
-#pragma unroll
-            for (int k = 0; k < BK; ++k) bf[c][k] = bwd_entry(cl - kRB, NK * wv + c, k + 1, bw);
+        for (int c = 0; c < NK; ++c) bwd_far(bf[c], cl - kRB, NK * wv + c);
         for (int b = nblk - 1; b >= 0; --b) {
             const int c0 = b * kRB, i = c0 + lane;
             double a[NK];
@@ -288,8 +303,7 @@ __device__ __forceinline__ void band_solve_lds(const T *__restrict__ fac, int n,
             for (int c = 0; c < NK; ++c) a[c] = bacc[c] + (double)bn[c] * xs[c0 + kRB + lane];
             const double s = wave_sums<NK>(a, lane);
             if ((lane & (64 / NK - 1)) == 0) sp[NK * wave + lane / (64 / NK)] = s;
-#pragma unroll
-            for (int c = 0; c < NK; ++c) bn[c] = bwd_entry(c0 - kRB, NK * wv + c, 0, bw);
+            bwd_near(bn, c0 - kRB);
             diag_store(false);
             diag_load(c0 - kRB, true);
             lds_barrier();
@@ -320,9 +334,7 @@ __device__ __forceinline__ void band_solve_lds(const T *__restrict__ fac, int n,
 #pragma unroll
                 for (int k = 0; k < BK; ++k) f += (double)bf[c][k] * xs[c0 + kRB * (k + 1) + lane];
                 bacc[c] = f;
-                const int bb = sizeof(T) == 8 ? opaque_s(bw) : bw;
-#pragma unroll
-                for (int k = 0; k < BK; ++k) bf[c][k] = bwd_entry(c0 - 2 * kRB, NK * wv + c, k + 1, bb);
+                bwd_far(bf[c], c0 - 2 * kRB, NK * wv + c);
                 __builtin_amdgcn_sched_barrier(0);
             }
             lds_barrier();
