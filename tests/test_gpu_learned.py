@@ -72,3 +72,28 @@ def test_learned_predictor_gains_on_held_out_roofs(ksh):
     assert used > 0.5 * steps * len(held)
     assert fb <= 0.2 * used
     assert jac_l < jac_b
+
+
+def test_network_of_another_shape_is_refused(ksh):
+    """The predictor reads 4 x 66 features and writes 64 mode coefficients and a load factor per
+    instance; a network of other widths passed through the C-ABI is refused before any step (the
+    host buffers are sized for those widths)."""
+    import ctypes as C
+    from gsstructuralanalysis_amd import _lib as L
+    from gsstructuralanalysis_amd.learned import WIDTHS
+    from gsstructuralanalysis_amd.problem import ShellProblem, run_alm_batch
+
+    class _Net:  # what run_alm_batch takes from a predictor: its handle
+        handle = C.c_void_p()
+
+    print(f"[case] learned predictor of widths other than {WIDTHS}", flush=True)
+    lib = L.load()
+    sps = [ShellProblem(g.imperfect_roof_instance(i, sizes=(4,))) for i in range(2)]
+    for widths in [(WIDTHS[0] + 1, 32, 32, WIDTHS[3]), (WIDTHS[0], 32, 32, WIDTHS[3] - 1)]:
+        net = _Net()
+        L.check(lib.ksh_mlp_create((C.c_int * 4)(*widths), 1, C.byref(net.handle)))
+        try:
+            with pytest.raises(L.KshError, match="must map"):
+                run_alm_batch(sps, 2, 6e-3, learned=net, **KW)
+        finally:
+            lib.ksh_mlp_destroy(net.handle)
