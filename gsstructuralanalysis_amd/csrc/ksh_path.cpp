@@ -334,6 +334,13 @@ private:
         int rc;
         // instances at a singular point advance one stage of its procedure
         std::vector<Inst *> live, sing;
+        // host wall time of each phase (profiling counters; GPU waits included)
+        double tq = now_s();
+        auto mark = [&](const char *name) {
+            const double t = now_s();
+            prof_count(name, (long long)((t - tq) * 1e6));
+            tq = t;
+        };
         for (Inst *I : all) (I->ph == Phase::Singular ? sing : live).push_back(I);
         if (!sing.empty() && (rc = singular_round(sing))) return rc;
         // ---- A: learned proposals ------------------------------------------------------------
@@ -344,6 +351,7 @@ private:
                 if (learned_usable(*I)) lp.push_back(I);
             }
         if (!lp.empty() && (rc = learned_predictor(lp))) return rc;
+        mark("round_predictor_us");
         // ---- B: tangents and their factors ---------------------------------------------------
         std::vector<TangentJob> tj;
         std::vector<Inst *> act;
@@ -355,6 +363,7 @@ private:
         }
         if (act.empty()) return KSH_SUCCESS;
         if ((rc = m_dev.tangent(tj))) return rc;
+        mark("round_tangent_us");
         {
             std::vector<Inst *> ok;
             for (Inst *I : act) {
@@ -390,6 +399,7 @@ private:
         }
         if ((rc = m_dev.vecops(neg))) return rc;
         if ((rc = m_dev.solve(sj))) return rc;
+        mark("round_solve_us");
         for (const SolveJob &q : sj) {
             Inst *I = static_cast<Inst *>(q.tag);
             ++I->solves;
@@ -406,8 +416,11 @@ private:
         }
         // ---- D: scalar logic and vector updates -------------------------------------------------
         if ((rc = update(act))) return rc;
+        mark("round_update_us");
         // ---- E: residual, norms, convergence ----------------------------------------------------
-        return residual_and_check(act, false);
+        rc = residual_and_check(act, false);
+        mark("round_residual_us");
+        return rc;
     }
 
     // ---- learned predictor (A) -------------------------------------------------------------------
