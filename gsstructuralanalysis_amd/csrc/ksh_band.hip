@@ -7,6 +7,8 @@
 // Layout: lower band, column major, ab[j*ld + (i-j)], ld = bw+1.
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+
 #include <cmath>
 #include <cstring>
 
@@ -74,9 +76,13 @@ __global__ void k_convert(TD *dst, const TS *src, size_t count)
     if (i < count) dst[i] = (TD)src[i];
 }
 
-// copy (or convert) of many bands in one launch: a 256-entry chunk per block, each block finds
-// its band by binary search over the chunk offsets
-__global__ void k_copy_batched(const CopyDesc *descs, int nband)
+// copy (or convert) of many bands in one launch: a chunk of kCopyBlk entries per block, each
+// thread kCopyPer consecutive entries with 16-byte loads and stores (when the chunk is whole and
+// the band pointers are 16-byte aligned; elementwise otherwise); each block finds its band by
+// binary search over the chunk offsets.  The conversion is the same per entry either way.
+constexpr int kCopyPer = 8, kCopyBlk = 256 * kCopyPer;
+
+__global__ __launch_bounds__(256) void k_copy_batched(const CopyDesc *descs, int nband)
 {
     const long long blk = blockIdx.x;
     int lo = 0, hi = nband - 1;
@@ -86,11 +92,48 @@ __global__ void k_copy_batched(const CopyDesc *descs, int nband)
         else hi = mid - 1;
     }
     const CopyDesc &d = descs[lo];
-    const size_t i = (size_t)(blk - d.blk0) * 256 + threadIdx.x;
-    if (i >= d.count) return;
-    const double v = d.src_prec == KSH_FP64 ? ((const double *)d.src)[i] : (double)((const float *)d.src)[i];
-    if (d.dst_prec == KSH_FP64) ((double *)d.dst)[i] = v;
-    else ((float *)d.dst)[i] = (float)v;
+    const size_t i0 = (size_t)(blk - d.blk0) * kCopyBlk + (size_t)threadIdx.x * kCopyPer;
+    if (i0 >= d.count) return;
+    const bool f64s = d.src_prec == KSH_FP64, f64d = d.dst_prec == KSH_FP64;
+    const bool vec = i0 + kCopyPer <= d.count && ((reinterpret_cast<uintptr_t>(d.src) | reinterpret_cast<uintptr_t>(d.dst)) & 15) == 0;
+    double v[kCopyPer];
+    if (vec) {
+        if (f64s) {
+            const double2 *p = reinterpret_cast<const double2 *>((const double *)d.src + i0);
+#pragma unroll
+            for (int q = 0; q < kCopyPer / 2; ++q) {
+                const double2 t = p[q];
+                v[2 * q] = t.x;
+                v[2 * q + 1] = t.y;
+            }
+        } else {
+            const float4 *p = reinterpret_cast<const float4 *>((const float *)d.src + i0);
+#pragma unroll
+            for (int q = 0; q < kCopyPer / 4; ++q) {
+                const float4 t = p[q];
+                v[4 * q] = t.x;
+                v[4 * q + 1] = t.y;
+                v[4 * q + 2] = t.z;
+                v[4 * q + 3] = t.w;
+            }
+        }
+        if (f64d) {
+            double2 *p = reinterpret_cast<double2 *>((double *)d.dst + i0);
+#pragma unroll
+            for (int q = 0; q < kCopyPer / 2; ++q) p[q] = make_double2(v[2 * q], v[2 * q + 1]);
+        } else {
+            float4 *p = reinterpret_cast<float4 *>((float *)d.dst + i0);
+#pragma unroll
+            for (int q = 0; q < kCopyPer / 4; ++q)
+                p[q] = make_float4((float)v[4 * q], (float)v[4 * q + 1], (float)v[4 * q + 2], (float)v[4 * q + 3]);
+        }
+        return;
+    }
+    for (size_t i = i0; i < d.count && i < i0 + kCopyPer; ++i) {
+        const double x = f64s ? ((const double *)d.src)[i] : (double)((const float *)d.src)[i];
+        if (f64d) ((double *)d.dst)[i] = x;
+        else ((float *)d.dst)[i] = (float)x;
+    }
 }
 
 __global__ void k_axpy(int n, double a, const double *x, double *y)
@@ -188,7 +231,7 @@ int band_copy_batch(hipStream_t s, void *d_descs, const std::vector<CopyDesc> &h
     long long blocks = 0;
     for (auto &d : h) {
         d.blk0 = blocks;
-        blocks += (long long)((d.count + 255) / 256);
+        blocks += (long long)((d.count + kCopyBlk - 1) / kCopyBlk);
     }
     if (blocks >= (1LL << 31)) return fail(KSH_OTHER_ERROR, "band copy of 2^31 blocks or more");
     KSH_HIP(hipMemcpyAsync(d_descs, h.data(), sizeof(CopyDesc) * h.size(), hipMemcpyHostToDevice, s));
