@@ -84,7 +84,8 @@ def test_network_of_another_shape_is_refused(ksh):
     from gsstructuralanalysis_amd.problem import ShellProblem, run_alm_batch
 
     class _Net:  # what run_alm_batch takes from a predictor: its handle
-        handle = C.c_void_p()
+        def __init__(self):
+            self.handle = C.c_void_p()
 
     print(f"[case] learned predictor of widths other than {WIDTHS}", flush=True)
     lib = L.load()
